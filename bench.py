@@ -1,0 +1,188 @@
+"""Throughput of the GFFT transform path on one MI355X.
+
+A step is one forward + inverse in-place C2C round trip of N = 2^24 FP32 points (the large
+plain-method workload), inputs resident in HBM, rotating over buffers totalling more than
+256 MB so that the Infinity Cache does not serve the input.  value = GFLOP/s with the
+reference's flop count 5 N log2 N per direction (test/gfft_performance.h:77).
+
+Also reported on the same line:
+  roofline      dominant kernel's algorithmic bytes (one read + one write of the array,
+                2 N b_c per pass) over its average duration from HIP events recorded on
+                the launch stream (GFFTX_FLAG_PROFILE), against 8 TB/s;
+  cpu_baseline  the GFFT restatement (OpenMP in-place path) on the host cores, bounded
+                sample, same input distribution;
+  fp64          the same round trip in FP64.
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--log2n", type=int, default=24)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    return ap.parse_args()
+
+
+def round_trip_run(torch, api, n, dtype, steps, warmup, nbuf, seed=24):
+    """Times `steps` forward+inverse round trips; returns dict with timings and profile."""
+    from gfft_amd import inputs
+
+    cdt = torch.complex64 if dtype == np.complex64 else torch.complex128
+    vt = api.COMPLEX_FLOAT if dtype == np.complex64 else api.COMPLEX_DOUBLE
+    x = inputs.complex_normal(n, seed).astype(dtype)
+    bufs = [torch.from_numpy(x).to("cuda") for _ in range(nbuf)]
+    flags = api.FLAG_PROFILE
+    pf = api.Plan(n, vt, api.DFT, api.IN_PLACE, flags=flags)
+    pi = api.Plan(n, vt, api.IDFT, api.IN_PLACE, flags=flags)
+    stream = torch.cuda.current_stream()
+    for i in range(warmup):
+        b = bufs[i % nbuf]
+        pf.fft(b)
+        pi.fft(b)
+    torch.cuda.synchronize()
+    # round-trip error after warmup round trips (each is the identity up to rounding)
+    rt_err = float(np.linalg.norm(bufs[0].cpu().numpy() - x) / np.linalg.norm(x))
+    for b in bufs:
+        b.copy_(torch.from_numpy(x))
+    pf.profile()
+    pi.profile()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record(stream)
+    for i in range(steps):
+        b = bufs[i % nbuf]
+        pf.fft(b)
+        pi.fft(b)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = start.elapsed_time(stop)
+    prof = {"fwd": pf.profile(), "inv": pi.profile()}
+    desc = pf.describe()
+    pf.close()
+    pi.close()
+    del bufs
+    torch.cuda.empty_cache()
+    return {"ms": ms, "wall_s": wall, "profile": prof, "rt_err": rt_err, "plan": desc}
+
+
+def cpu_baseline(n_log2: int):
+    """GFFT restatement, OpenMP in-place path (src/gfftomp.h:105-144), FP32 round trip of
+    2^n_log2 points, best of 3 (test/gfft_performance.h:104-123), wall clock."""
+    from gfft_amd import inputs
+    from oracle import oracle as O
+
+    ncores = os.cpu_count() or 1
+    nt = 1
+    while nt * 2 <= min(ncores, 16):
+        nt *= 2
+    n = 1 << n_log2
+    x = inputs.complex_normal(n, 24).astype(np.complex64)
+    best = 1e30
+    for _ in range(3):
+        t0 = time.perf_counter()
+        y = O.gfft_inplace(x, 1, nt)
+        O.gfft_inplace(y, -1, nt)
+        best = min(best, time.perf_counter() - t0)
+    flops = 2 * 5 * n * n_log2
+    return {"value": round(flops / best / 1e9, 3), "unit": "GFLOP/s", "cores": nt, "kind": "port",
+            "sample": f"3 round trips (forward+inverse, in place) of N=2^{n_log2} FP32 with the "
+                      f"GFFT OpenMP restatement (NT={nt}, serial combine and swap), best of 3 "
+                      f"= {best:.2f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the transform shards as described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from gfft_amd import api
+
+    api.lib()
+    n = 1 << args.log2n
+    # > 256 MB of rotating input buffers
+    nbuf32 = max(2, math.ceil(512 * 2 ** 20 / (n * 8)))
+    r32 = round_trip_run(torch, api, n, np.complex64, args.steps, args.warmup, nbuf32)
+    flops_step = 2 * 5 * n * args.log2n
+    ms_step = r32["ms"] / args.steps
+    value = flops_step / (ms_step * 1e-3) / 1e9
+
+    # dominant kernel of the FP32 round trip
+    kern = {}
+    for direction in ("fwd", "inv"):
+        for i, (name, tot, cnt) in enumerate(r32["profile"][direction]):
+            key = f"{direction}:pass{i}:{name}"
+            kern[key] = (tot, cnt)
+    dom_key = max(kern, key=lambda k: kern[k][0])
+    dom_tot, dom_cnt = kern[dom_key]
+    dom_avg_s = dom_tot / dom_cnt * 1e-3
+    bytes_per_launch = 2 * n * 8
+    achieved = bytes_per_launch / dom_avg_s
+    roofline = {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                "kernel": dom_key, "avg_us": round(dom_avg_s * 1e6, 2),
+                "bytes_per_launch": bytes_per_launch}
+    per_kernel = {k: {"avg_us": round(v[0] / v[1] * 1e3, 2),
+                      "GBps_alg": round(bytes_per_launch / (v[0] / v[1] * 1e-3) / 1e9, 1)}
+                  for k, v in kern.items()}
+
+    out = {
+        "metric": "C2C FFT GFLOP/s + %HBM peak: MFMA split-prec vs plain FP32 at relL2<=2e-7",
+        "value": round(value, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: re,im ~ N(0,1) from SplitMix64 seed 24",
+        "config": {"workload": f"C2C forward+inverse round trip, N=2^{args.log2n}, FP32, in place, "
+                               f"multi-pass Stockham, {nbuf32} rotating buffers",
+                   "n": n, "passes": json.loads(r32["plan"])["passes"].__len__(),
+                   "round_trip_relL2": r32["rt_err"]},
+        "alg_GBps": round(2 * 2 * n * 8 / (ms_step * 1e-3) / 1e9, 1),
+        "roofline": roofline,
+        "kernels": per_kernel,
+    }
+    if not args.no_fp64:
+        nbuf64 = max(2, math.ceil(512 * 2 ** 20 / (n * 16)))
+        r64 = round_trip_run(torch, api, n, np.complex128, max(10, args.steps // 2),
+                             args.warmup, nbuf64)
+        ms64 = r64["ms"] / max(10, args.steps // 2)
+        out["fp64"] = {"GFLOPs": round(flops_step / (ms64 * 1e-3) / 1e9, 2),
+                       "ms_per_step": round(ms64, 4), "round_trip_relL2": r64["rt_err"],
+                       "alg_GBps": round(2 * 2 * n * 16 / (ms64 * 1e-3) / 1e9, 1)}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.log2n)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,149 @@
+// kernels.hip -- instantiated FFT kernels for gfx950 and their host-visible registry.
+#include "registry.h"
+
+namespace gfftx {
+
+template <class C> static hipError_t launch_tile(const TileArgs& a, dim3 grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(tile_kernel<C>, grid, dim3(C::WG), 0, s, a);
+    return hipGetLastError();
+}
+
+#define GFFTX_CFG(TYPE, L, NB, WG, LT, ST, ...) TileCfg<TYPE, L, NB, WG, LT, ST, __VA_ARGS__>
+#define GFFTX_TILE(PREC, TYPE, L, NB, WG, LT, ST, ...)                                         \
+    {PREC,                                                                                     \
+     L,                                                                                        \
+     NB,                                                                                       \
+     WG,                                                                                       \
+     LT,                                                                                       \
+     ST,                                                                                       \
+     GFFTX_CFG(TYPE, L, NB, WG, LT, ST, __VA_ARGS__)::NS,                                     \
+     GFFTX_CFG(TYPE, L, NB, WG, LT, ST, __VA_ARGS__)::R,                                      \
+     (int)(GFFTX_CFG(TYPE, L, NB, WG, LT, ST, __VA_ARGS__)::LDS_ELEMS * sizeof(V2<TYPE>)),    \
+     &launch_tile<GFFTX_CFG(TYPE, L, NB, WG, LT, ST, __VA_ARGS__)>,                           \
+     "tile_" #TYPE "_L" #L "_nb" #NB "_wg" #WG "_" #LT #ST}
+
+// Batched single-pass (E->E) tiles: TPT = L/16 threads per sequence, 256-thread workgroups.
+#define GFFTX_BATCH_SET(PREC, TYPE)                                          \
+    GFFTX_TILE(PREC, TYPE, 16, 64, 64, false, false, 16),                   \
+    GFFTX_TILE(PREC, TYPE, 32, 64, 128, false, false, 16, 2),               \
+    GFFTX_TILE(PREC, TYPE, 64, 64, 256, false, false, 16, 4),               \
+    GFFTX_TILE(PREC, TYPE, 128, 32, 256, false, false, 16, 8),              \
+    GFFTX_TILE(PREC, TYPE, 256, 16, 256, false, false, 16, 16),             \
+    GFFTX_TILE(PREC, TYPE, 512, 8, 256, false, false, 16, 16, 2),           \
+    GFFTX_TILE(PREC, TYPE, 1024, 4, 256, false, false, 16, 16, 4),          \
+    GFFTX_TILE(PREC, TYPE, 2048, 2, 256, false, false, 16, 16, 8),          \
+    GFFTX_TILE(PREC, TYPE, 4096, 1, 256, false, false, 16, 16, 16),         \
+    GFFTX_TILE(PREC, TYPE, 8192, 1, 512, false, false, 16, 16, 16, 2)
+
+// Pass tiles for multi-pass transforms: first pass (T->E) and later passes (T->T).
+#define GFFTX_PASS_SET(PREC, TYPE, NB4096, WG4096, NB1024, WG1024)                          \
+    GFFTX_TILE(PREC, TYPE, 256, 16, 256, true, false, 16, 16),                             \
+    GFFTX_TILE(PREC, TYPE, 256, 16, 256, true, true, 16, 16),                              \
+    GFFTX_TILE(PREC, TYPE, 1024, NB1024, WG1024, true, false, 16, 16, 4),                  \
+    GFFTX_TILE(PREC, TYPE, 1024, NB1024, WG1024, true, true, 16, 16, 4),                   \
+    GFFTX_TILE(PREC, TYPE, 4096, NB4096, WG4096, true, false, 16, 16, 16),                 \
+    GFFTX_TILE(PREC, TYPE, 4096, NB4096, WG4096, true, true, 16, 16, 16)
+
+// Host table.  Not const: a const namespace-scope table would be emitted into the device
+// image as well; both compilation passes still instantiate every kernel listed here.
+TileEntry kTiles[] = {
+    GFFTX_BATCH_SET(PREC_F32, float),
+    GFFTX_TILE(PREC_F32, float, 16384, 1, 1024, false, false, 16, 16, 16, 4),
+    GFFTX_PASS_SET(PREC_F32, float, 4, 1024, 8, 512),
+    GFFTX_BATCH_SET(PREC_F64, double),
+    GFFTX_PASS_SET(PREC_F64, double, 2, 512, 4, 256),
+};
+const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
+
+// ---------------------------------------------------------------------------------------
+// Real-transform split pass.  Forward (post-pass of RDFT) for k in [1, n/2]:
+//   h1 = (Z_k + conj Z_{n-k})/2,  h2 = -i (Z_k - conj Z_{n-k})/2,  w = W_{2n}^k
+//   X_k = h1 + w h2,  X_{n-k} = conj(h1 - w h2)
+// and X_0 = Re Z_0 + Im Z_0, X_n = Re Z_0 - Im Z_0.  Inverse (pre-pass of IRDFT) rebuilds
+// Z_k = h1 + i conj(w) (X_k - conj X_{n-k})/2, Z_{n-k} = conj(h1 - i conj(w) (..)/2) with
+// h1 = (X_k + conj X_{n-k})/2, and Z_0 = ((X_0 + X_n)/2, (X_0 - X_n)/2).  These are the
+// relations GFFT's Separate evaluates with a twiddle recurrence (src/gfftswap.h:246-280).
+template <typename T> __global__ __launch_bounds__(256) void split_kernel(SplitArgs a)
+{
+    using V = V2<T>;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = a.n;
+    if (k > n / 2) return;
+    V* d = reinterpret_cast<V*>(a.data) + (long long)blockIdx.y * a.bdist;
+    const V* tw = reinterpret_cast<const V*>(a.tw);
+    const T half = (T)0.5;
+    if (k == 0) {
+        const V z0 = d[0];
+        if (!a.inverse) {
+            const T x0 = z0.x + z0.y, xn = z0.x - z0.y;
+            if (a.bins) {
+                d[0] = V{x0, 0};
+                d[n] = V{xn, 0};
+            } else {
+                d[0] = V{x0, xn};
+            }
+        } else {
+            const T x0 = z0.x;
+            const T xn = a.bins ? d[n].x : z0.y;
+            d[0] = V{half * (x0 + xn), half * (x0 - xn)};
+        }
+        return;
+    }
+    const V A = d[k];
+    const V B = d[n - k];
+    const V w = tw[k];
+    const V h1 = V{half * (A.x + B.x), half * (A.y - B.y)};
+    const V dd = V{half * (A.x - B.x), half * (A.y + B.y)};  // (A - conj B)/2
+    V p;
+    if (!a.inverse) {
+        const V h2 = V{dd.y, -dd.x};  // -i * dd
+        p = cmul(w, h2);
+    } else {
+        const V cw = V{w.x, -w.y};
+        const V t = cmul(cw, dd);
+        p = V{-t.y, t.x};  // i * conj(w) * dd
+    }
+    d[k] = V{h1.x + p.x, h1.y + p.y};
+    d[n - k] = V{h1.x - p.x, -(h1.y - p.y)};
+}
+
+hipError_t launch_split(int prec, const SplitArgs& a, hipStream_t s)
+{
+    const long long work = a.n / 2 + 1;
+    dim3 grid((unsigned)((work + 255) / 256), (unsigned)a.nbatch);
+    if (prec == PREC_F64) hipLaunchKernelGGL(split_kernel<double>, grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(split_kernel<float>, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------
+// Digit reversal dst[rev(k)] = src[k] (the permutation GFFTswap2 applies in place).
+template <typename T>
+__global__ __launch_bounds__(256) void digit_reverse_kernel(const V2<T>* src, V2<T>* dst, long long radix,
+                                                            int digits, long long n)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    long long x = k, r = 0;
+    for (int i = 0; i < digits; ++i) {
+        r = r * radix + x % radix;
+        x /= radix;
+    }
+    dst[r] = src[k];
+}
+
+hipError_t launch_digit_reverse(int prec, const void* src, void* dst, long long radix, int digits, long long n,
+                                hipStream_t s)
+{
+    dim3 grid((unsigned)((n + 255) / 256));
+    if (prec == PREC_F64)
+        hipLaunchKernelGGL(digit_reverse_kernel<double>, grid, dim3(256), 0, s, (const double2*)src,
+                           (double2*)dst, radix, digits, n);
+    else
+        hipLaunchKernelGGL(digit_reverse_kernel<float>, grid, dim3(256), 0, s, (const float2*)src,
+                           (float2*)dst, radix, digits, n);
+    return hipGetLastError();
+}
+
+}  // namespace gfftx

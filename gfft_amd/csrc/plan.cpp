@@ -1,0 +1,767 @@
+// plan.cpp -- planner and C ABI of the gfx950 FFT engine.
+//
+// A plan is a list of passes over device buffers.  A transform whose length fits one LDS
+// tile is one batched pass; a longer one is a Stockham decomposition into "super radices"
+// N = P_0 * P_1 * ... each executed as one pass of the tile kernel (see tile_kernel.h).
+// Twiddles are generated here once per plan from long-double evaluations with octant
+// argument reduction and stored in the working precision (stage tables) or FP64 (pass
+// tables).  GFFT evaluates the same roots W_N = exp(-2 pi i/N) at compile time to 18
+// digits (src/metaroot.h:104-120) and then runs a recurrence at run time
+// (src/gfftalg.h:249-261); tables avoid the recurrence's error growth.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/gfftx.h"
+#include "registry.h"
+
+using namespace gfftx;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(GFFTX_ERR_DEVICE, "%s: %s", #expr, hipGetErrorString(e_));      \
+    } while (0)
+
+// exp(-2 pi i m / N) in long double, m reduced mod N and folded into the first octant.
+void root_ld(long long m, long long N, long double* c, long double* s)
+{
+    const long double PI = 3.141592653589793238462643383279502884L;
+    m %= N;
+    if (m < 0) m += N;
+    // angle = (pi/4) * (8m/N); octant o and remainder in units of N/8
+    const long double x8 = 8.0L * (long double)m;
+    long long o = (long long)(x8 / (long double)N);
+    if (o > 7) o = 7;
+    const long double r = x8 - (long double)o * (long double)N;
+    long double cc, ss;
+    const bool odd = (o & 1) != 0;
+    const long double a = (PI / 4.0L) * (odd ? ((long double)N - r) : r) / (long double)N;
+    const long double ca = cosl(a), sa = sinl(a);
+    // cos/sin of the full angle from the folded one
+    switch (o) {
+        case 0: cc = ca; ss = sa; break;
+        case 1: cc = sa; ss = ca; break;
+        case 2: cc = -sa; ss = ca; break;
+        case 3: cc = -ca; ss = sa; break;
+        case 4: cc = -ca; ss = -sa; break;
+        case 5: cc = -sa; ss = -ca; break;
+        case 6: cc = sa; ss = -ca; break;
+        default: cc = ca; ss = -sa; break;
+    }
+    *c = cc;
+    *s = -ss;
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Pass {
+    const TileEntry* k = nullptr;
+    long long ntrans = 0, its = 0, ies = 0, ns_pass = 1, bdist = 0;
+    unsigned batch = 1;
+    void* stage_tw = nullptr;
+    double2* ptw_lo = nullptr;
+    double2* ptw_hi = nullptr;
+    int ptw_shift = 0;
+    long long ptw_m = 0;
+};
+
+size_t elem_bytes(int prec) { return prec == PREC_F64 ? 16 : 8; }
+
+const TileEntry* find_tile(int prec, int L, bool loadT, bool storeT)
+{
+    for (int i = 0; i < kNumTiles; ++i) {
+        const TileEntry& e = kTiles[i];
+        if (e.prec == prec && e.L == L && e.loadT == loadT && e.storeT == storeT) return &e;
+    }
+    return nullptr;
+}
+
+int max_single_tile(int prec)
+{
+    int best = 0;
+    for (int i = 0; i < kNumTiles; ++i)
+        if (kTiles[i].prec == prec && !kTiles[i].loadT && !kTiles[i].storeT) best = std::max(best, kTiles[i].L);
+    return best;
+}
+
+bool only_235(unsigned long long n)
+{
+    for (unsigned long long p : {2ull, 3ull, 5ull})
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
+
+// Split n into pass lengths available as T-mode tiles (largest first).  An override list
+// (GFFTX_PASSES="a,b,c") is honoured when it multiplies to n.
+bool choose_passes(int prec, unsigned long long n, std::vector<int>& out)
+{
+    out.clear();
+    if (const char* env = getenv("GFFTX_PASSES")) {
+        std::vector<int> v;
+        unsigned long long prod = 1;
+        const char* p = env;
+        while (*p) {
+            const long x = strtol(p, const_cast<char**>(&p), 10);
+            if (x <= 1) break;
+            v.push_back((int)x);
+            prod *= (unsigned long long)x;
+            if (*p == ',') ++p;
+            else break;
+        }
+        bool ok = prod == n && v.size() >= 2;
+        for (size_t i = 0; ok && i < v.size(); ++i)
+            ok = find_tile(prec, v[i], true, i == 0 ? false : true) != nullptr;
+        if (ok) {
+            out = v;
+            return true;
+        }
+    }
+    std::vector<int> lens;
+    for (int i = 0; i < kNumTiles; ++i)
+        if (kTiles[i].prec == prec && kTiles[i].loadT && kTiles[i].storeT) lens.push_back(kTiles[i].L);
+    std::sort(lens.begin(), lens.end());
+    lens.erase(std::unique(lens.begin(), lens.end()), lens.end());
+    // prefer the fewest passes, then the most balanced split
+    for (int npass = 2; npass <= 4; ++npass) {
+        std::vector<int> best;
+        double best_score = 1e300;
+        std::vector<int> cur(npass, 0);
+        // enumerate non-increasing sequences of lens with product n
+        std::function<void(int, unsigned long long, int)> rec = [&](int pos, unsigned long long rem, int maxi) {
+            if (pos == npass) {
+                if (rem == 1) {
+                    double mx = 0, mn = 1e300;
+                    for (int v : cur) { mx = std::max(mx, (double)v); mn = std::min(mn, (double)v); }
+                    const double score = mx / mn;
+                    if (score < best_score) { best_score = score; best = cur; }
+                }
+                return;
+            }
+            for (int i = maxi; i >= 0; --i) {
+                if (rem % (unsigned long long)lens[i]) continue;
+                cur[pos] = lens[i];
+                rec(pos + 1, rem / lens[i], i);
+            }
+        };
+        rec(0, n, (int)lens.size() - 1);
+        if (!best.empty()) {
+            bool ok = find_tile(prec, best[0], true, false) != nullptr;
+            if (ok) {
+                out = best;
+                return true;
+            }
+        }
+    }
+    return false;
+}
+
+}  // namespace
+
+struct gfftx_plan_s {
+    int device = 0;
+    unsigned long long n = 0;      // complex length of one transform
+    unsigned long long batch = 1;
+    int vtype = 0, ttype = 0, place = 0;
+    uint32_t flags = 0;
+    int prec = PREC_F64;
+    bool real = false;
+    bool inverse = false;
+    unsigned long long dist = 0;   // complex elements between batch items
+    std::vector<Pass> passes;
+    std::vector<void*> allocs;     // device allocations owned by the plan
+    void* scratch[2] = {nullptr, nullptr};
+    void* split_tw = nullptr;
+    void* real_work = nullptr;     // IRDFT out-of-place: split runs on a copy of src
+    DevBuf stage_in, stage_out;    // staging for host pointers
+    std::string desc;
+    // GFFTX_FLAG_PROFILE: one (start, stop) event pair per launch slot and execution
+    struct Slot {
+        std::string name;
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+    };
+    std::vector<Slot> slots;
+};
+
+namespace {
+
+int dev_alloc(gfftx_plan_s* p, void** out, size_t bytes)
+{
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(out, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(GFFTX_ERR_ALLOC, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    p->allocs.push_back(*out);
+    return GFFTX_SUCCESS;
+}
+
+template <typename T> void pack_stage_table(const TileEntry* k, std::vector<T>& out)
+{
+    // stage s >= 1: rows k in [0, Ns), entries r in [0, R): W_{Ns*R}^{k*r}
+    out.clear();
+    long long ns = k->radices[0];
+    for (int s = 1; s < k->nrad; ++s) {
+        const long long R = k->radices[s];
+        for (long long kk = 0; kk < ns; ++kk)
+            for (long long r = 0; r < R; ++r) {
+                long double c, si;
+                root_ld(kk * r, ns * R, &c, &si);
+                out.push_back((T)c);
+                out.push_back((T)si);
+            }
+        ns *= R;
+    }
+}
+
+int upload_stage_table(gfftx_plan_s* p, const TileEntry* k, void** out)
+{
+    *out = nullptr;
+    if (k->nrad <= 1) return GFFTX_SUCCESS;
+    if (p->prec == PREC_F64) {
+        std::vector<double> t;
+        pack_stage_table(k, t);
+        int rc = dev_alloc(p, out, t.size() * sizeof(double));
+        if (rc) return rc;
+        HIP_TRY(hipMemcpy(*out, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> t;
+        pack_stage_table(k, t);
+        int rc = dev_alloc(p, out, t.size() * sizeof(float));
+        if (rc) return rc;
+        HIP_TRY(hipMemcpy(*out, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    return GFFTX_SUCCESS;
+}
+
+int upload_pass_table(gfftx_plan_s* p, Pass& ps, long long M)
+{
+    int b = 0;
+    while ((1LL << (2 * b)) < M) ++b;  // lo table 2^b ~ sqrt(M)
+    const long long nlo = 1LL << b, nhi = (M + nlo - 1) / nlo + 1;
+    std::vector<double> lo(2 * nlo), hi(2 * nhi);
+    for (long long m = 0; m < nlo; ++m) {
+        long double c, s;
+        root_ld(m, M, &c, &s);
+        lo[2 * m] = (double)c;
+        lo[2 * m + 1] = (double)s;
+    }
+    for (long long h = 0; h < nhi; ++h) {
+        long double c, s;
+        root_ld(h * nlo, M, &c, &s);
+        hi[2 * h] = (double)c;
+        hi[2 * h + 1] = (double)s;
+    }
+    void* dlo = nullptr;
+    void* dhi = nullptr;
+    int rc = dev_alloc(p, &dlo, lo.size() * sizeof(double));
+    if (rc) return rc;
+    rc = dev_alloc(p, &dhi, hi.size() * sizeof(double));
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(dlo, lo.data(), lo.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dhi, hi.data(), hi.size() * sizeof(double), hipMemcpyHostToDevice));
+    ps.ptw_lo = (double2*)dlo;
+    ps.ptw_hi = (double2*)dhi;
+    ps.ptw_shift = b;
+    ps.ptw_m = M;
+    return GFFTX_SUCCESS;
+}
+
+int build_c2c_passes(gfftx_plan_s* p)
+{
+    const unsigned long long n = p->n;
+    const int maxL = max_single_tile(p->prec);
+    char buf[256];
+    p->desc = "{\"n\":" + std::to_string(n) + ",\"batch\":" + std::to_string(p->batch) +
+              ",\"precision\":\"" + (p->prec == PREC_F64 ? "f64" : "f32") + "\",\"passes\":[";
+    if (n <= (unsigned long long)maxL) {
+        const TileEntry* k = find_tile(p->prec, (int)n, false, false);
+        if (!k) return fail(GFFTX_ERR_UNSUPPORTED, "no single-pass kernel for n=%llu", n);
+        Pass ps;
+        ps.k = k;
+        ps.ntrans = (long long)p->batch;
+        ps.its = (long long)p->dist;
+        ps.ies = 1;
+        ps.ns_pass = 1;
+        ps.bdist = 0;
+        ps.batch = 1;
+        int rc = upload_stage_table(p, k, &ps.stage_tw);
+        if (rc) return rc;
+        p->passes.push_back(ps);
+        snprintf(buf, sizeof(buf), "{\"kernel\":\"%s\",\"L\":%d,\"ntrans\":%lld}", k->name, k->L, ps.ntrans);
+        p->desc += buf;
+    } else {
+        std::vector<int> lens;
+        if (!choose_passes(p->prec, n, lens))
+            return fail(GFFTX_ERR_UNSUPPORTED, "no multi-pass decomposition for n=%llu", n);
+        long long ns = 1;
+        for (size_t i = 0; i < lens.size(); ++i) {
+            const int L = lens[i];
+            const TileEntry* k = find_tile(p->prec, L, true, i > 0);
+            if (!k) return fail(GFFTX_ERR_UNSUPPORTED, "no pass kernel L=%d", L);
+            Pass ps;
+            ps.k = k;
+            ps.ntrans = (long long)(n / L);
+            ps.its = 1;
+            ps.ies = (long long)(n / L);
+            ps.ns_pass = ns;
+            ps.bdist = (long long)p->dist;
+            ps.batch = (unsigned)p->batch;
+            int rc = upload_stage_table(p, k, &ps.stage_tw);
+            if (rc) return rc;
+            if (ns > 1) {
+                rc = upload_pass_table(p, ps, ns * L);
+                if (rc) return rc;
+            }
+            p->passes.push_back(ps);
+            snprintf(buf, sizeof(buf), "%s{\"kernel\":\"%s\",\"L\":%d,\"ns\":%lld,\"ntrans\":%lld}",
+                     i ? "," : "", k->name, L, ns, ps.ntrans);
+            p->desc += buf;
+            ns *= L;
+        }
+    }
+    p->desc += "]}";
+    return GFFTX_SUCCESS;
+}
+
+int upload_split_table(gfftx_plan_s* p)
+{
+    const long long n = (long long)p->n;
+    const long long cnt = n / 2 + 1;
+    if (p->prec == PREC_F64) {
+        std::vector<double> t(2 * cnt);
+        for (long long k = 0; k < cnt; ++k) {
+            long double c, s;
+            root_ld(k, 2 * n, &c, &s);
+            t[2 * k] = (double)c;
+            t[2 * k + 1] = (double)s;
+        }
+        int rc = dev_alloc(p, &p->split_tw, t.size() * sizeof(double));
+        if (rc) return rc;
+        HIP_TRY(hipMemcpy(p->split_tw, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> t(2 * cnt);
+        for (long long k = 0; k < cnt; ++k) {
+            long double c, s;
+            root_ld(k, 2 * n, &c, &s);
+            t[2 * k] = (float)c;
+            t[2 * k + 1] = (float)s;
+        }
+        int rc = dev_alloc(p, &p->split_tw, t.size() * sizeof(float));
+        if (rc) return rc;
+        HIP_TRY(hipMemcpy(p->split_tw, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    return GFFTX_SUCCESS;
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    bool ok = false;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) {
+            (void)hipGetLastError();
+            prev = -1;
+        }
+        ok = hipSetDevice(dev) == hipSuccess;
+        if (!ok) (void)hipGetLastError();
+    }
+    ~DeviceGuard()
+    {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+bool is_device_ptr(const void* ptr)
+{
+    hipPointerAttribute_t attr;
+    hipError_t e = hipPointerGetAttributes(&attr, ptr);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+// Event pair for launch slot `slot` (GFFTX_FLAG_PROFILE); returns false when not profiling.
+bool prof_begin(gfftx_plan_s* p, size_t slot, const char* name, hipStream_t s, std::pair<hipEvent_t, hipEvent_t>* ev)
+{
+    if (!(p->flags & GFFTX_FLAG_PROFILE)) return false;
+    if (p->slots.size() <= slot) p->slots.resize(slot + 1);
+    gfftx_plan_s::Slot& sl = p->slots[slot];
+    sl.name = name;
+    if (sl.pool.empty()) {
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        sl.pool.push_back({a, b});
+    }
+    *ev = sl.pool.back();
+    sl.pool.pop_back();
+    (void)hipEventRecord(ev->first, s);
+    return true;
+}
+
+void prof_end(gfftx_plan_s* p, size_t slot, hipStream_t s, const std::pair<hipEvent_t, hipEvent_t>& ev)
+{
+    (void)hipEventRecord(ev.second, s);
+    p->slots[slot].pending.push_back(ev);
+}
+
+int run_passes(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
+{
+    const size_t m = p->passes.size();
+    // buffer for the output of pass i (i < m-1): alternate so that no pass is in place and
+    // the caller's input is never written unless it is also the output
+    std::vector<void*> dst(m);
+    dst[m - 1] = out;
+    for (size_t i = m - 1; i-- > 0;) {
+        void* next_src_dst = dst[i + 1];
+        void* cand = (next_src_dst == p->scratch[0]) ? out : p->scratch[0];
+        if (cand == in || cand == next_src_dst) cand = p->scratch[1];
+        dst[i] = cand;
+    }
+    const void* src = in;
+    for (size_t i = 0; i < m; ++i) {
+        const Pass& ps = p->passes[i];
+        TileArgs a;
+        memset(&a, 0, sizeof(a));
+        a.in = src;
+        a.out = dst[i];
+        a.ntrans = ps.ntrans;
+        a.its = ps.its;
+        a.ies = ps.ies;
+        a.ns_pass = ps.ns_pass;
+        a.bdist = ps.bdist;
+        a.tw = ps.stage_tw;
+        a.ptw_lo = ps.ptw_lo;
+        a.ptw_hi = ps.ptw_hi;
+        a.ptw_shift = ps.ptw_shift;
+        a.conj_in = (p->inverse && i == 0) ? 1 : 0;
+        a.conj_out = (p->inverse && i == m - 1) ? 1 : 0;
+        a.scale = (p->inverse && i == m - 1) ? 1.0 / (double)p->n : 1.0;
+        const long long nb = ps.k->nb;
+        dim3 grid((unsigned)((ps.ntrans + nb - 1) / nb), ps.batch);
+        std::pair<hipEvent_t, hipEvent_t> ev;
+        const bool prof = prof_begin(p, i, ps.k->name, s, &ev);
+        hipError_t e = ps.k->launch(a, grid, s);
+        if (prof) prof_end(p, i, s, ev);
+        if (e != hipSuccess) return fail(GFFTX_ERR_LAUNCH, "%s: %s", ps.k->name, hipGetErrorString(e));
+        src = dst[i];
+    }
+    return GFFTX_SUCCESS;
+}
+
+int run_split(gfftx_plan_s* p, void* data, hipStream_t s)
+{
+    SplitArgs a;
+    a.data = data;
+    a.n = (long long)p->n;
+    a.bdist = (long long)p->dist;
+    a.nbatch = (long long)p->batch;
+    a.tw = p->split_tw;
+    a.inverse = p->inverse ? 1 : 0;
+    a.bins = (p->flags & GFFTX_FLAG_R2C_BINS) ? 1 : 0;
+    std::pair<hipEvent_t, hipEvent_t> ev;
+    const size_t slot = p->passes.size();
+    const bool prof = prof_begin(p, slot, "split_kernel", s, &ev);
+    hipError_t e = launch_split(p->prec, a, s);
+    if (prof) prof_end(p, slot, s, ev);
+    if (e != hipSuccess) return fail(GFFTX_ERR_LAUNCH, "split: %s", hipGetErrorString(e));
+    return GFFTX_SUCCESS;
+}
+
+// Full transform on device pointers.  Real forward: complex FFT then split on the output.
+// Real inverse: split on a copy of the input (the input of an out-of-place call is left
+// unchanged), then the inverse complex FFT.
+int exec_device(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
+{
+    if (!p->real) return run_passes(p, in, out, s);
+    const size_t bytes = (size_t)p->dist * p->batch * elem_bytes(p->prec);
+    if (!p->inverse) {
+        int rc = run_passes(p, in, out, s);
+        if (rc) return rc;
+        return run_split(p, out, s);
+    }
+    void* work = out;
+    if (in != out) {
+        if (!p->real_work) {
+            int rc = dev_alloc(p, &p->real_work, bytes);
+            if (rc) return rc;
+        }
+        work = p->real_work;
+        HIP_TRY(hipMemcpyAsync(work, in, bytes, hipMemcpyDeviceToDevice, s));
+    }
+    int rc = run_split(p, work, s);
+    if (rc) return rc;
+    return run_passes(p, work, out, s);
+}
+
+int exec_any(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
+{
+    DeviceGuard g(p->device);
+    if (!g.ok) return fail(GFFTX_ERR_DEVICE, "cannot select device %d", p->device);
+    const bool din = is_device_ptr(in), dout = is_device_ptr(out);
+    // complex elements the caller's arrays hold: the real side of a real transform has n
+    // per item, the complex side dist (n, or n+1 with the bins layout)
+    const size_t full = (size_t)p->dist * p->batch, real_side = (size_t)p->dist * (p->batch - 1) + p->n;
+    const size_t eb = elem_bytes(p->prec);
+    size_t in_bytes = full * eb, out_bytes = full * eb;
+    if (p->real && in != out) {
+        if (p->inverse) out_bytes = real_side * eb;
+        else in_bytes = real_side * eb;
+    }
+    if (din && dout) return exec_device(p, in, out, s);
+    // host staging: synchronous, as GFFT's fft() is
+    if (!p->stage_in.p) {
+        int rc = dev_alloc(p, &p->stage_in.p, full * eb);
+        if (rc) return rc;
+        p->stage_in.bytes = full * eb;
+    }
+    void* dev_in = din ? const_cast<void*>(in) : p->stage_in.p;
+    void* dev_out;
+    if (in == out) {
+        dev_out = dev_in;
+    } else if (dout) {
+        dev_out = out;
+    } else {
+        if (!p->stage_out.p) {
+            int rc = dev_alloc(p, &p->stage_out.p, full * eb);
+            if (rc) return rc;
+            p->stage_out.bytes = full * eb;
+        }
+        dev_out = p->stage_out.p;
+    }
+    if (!din) HIP_TRY(hipMemcpyAsync(dev_in, in, in_bytes, hipMemcpyHostToDevice, s));
+    int rc = exec_device(p, dev_in, dev_out, s);
+    if (rc) return rc;
+    if (!dout) HIP_TRY(hipMemcpyAsync(out, dev_out, out_bytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return GFFTX_SUCCESS;
+}
+
+int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttype, int place, int device,
+                uint32_t flags)
+{
+    if (!out) return fail(GFFTX_ERR_INVALID_ARG, "plan pointer is NULL");
+    *out = nullptr;
+    if (n == 0 || batch == 0) return fail(GFFTX_ERR_INVALID_ARG, "n and batch must be positive");
+    if (vtype < 0 || vtype > 3) return fail(GFFTX_ERR_INVALID_ARG, "unknown value type %d", vtype);
+    if (ttype < 0 || ttype > 3) return fail(GFFTX_ERR_INVALID_ARG, "unknown transform type %d", ttype);
+    if (place != GFFTX_IN_PLACE && place != GFFTX_OUT_OF_PLACE)
+        return fail(GFFTX_ERR_INVALID_ARG, "unknown place %d", place);
+    if (flags & ~(GFFTX_FLAG_R2C_BINS | GFFTX_FLAG_SPLIT_MFMA | GFFTX_FLAG_NO_AUTOTUNE | GFFTX_FLAG_GRAPH))
+        return fail(GFFTX_ERR_INVALID_ARG, "unknown flags 0x%x", flags);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        return fail(GFFTX_ERR_DEVICE, "no HIP device available");
+    }
+    if (device < 0 || device >= ndev) return fail(GFFTX_ERR_DEVICE, "device %d out of range (%d)", device, ndev);
+    const bool real = ttype == GFFTX_RDFT || ttype == GFFTX_IRDFT;
+    if (!only_235(n)) return fail(GFFTX_ERR_UNSUPPORTED, "length %llu has a prime factor above 5", (unsigned long long)n);
+    if (real && (n % 2))
+        return fail(GFFTX_ERR_UNSUPPORTED, "real transforms need an even complex length (n=%llu)",
+                    (unsigned long long)n);
+    if ((flags & GFFTX_FLAG_R2C_BINS) && !real)
+        return fail(GFFTX_ERR_INVALID_ARG, "GFFTX_FLAG_R2C_BINS applies to RDFT/IRDFT only");
+    if ((flags & GFFTX_FLAG_R2C_BINS) && batch != 1)
+        return fail(GFFTX_ERR_UNSUPPORTED, "the bins layout is available for batch 1 only");
+
+    DeviceGuard g(device);
+    if (!g.ok) return fail(GFFTX_ERR_DEVICE, "cannot select device %d", device);
+    gfftx_plan_s* p = new (std::nothrow) gfftx_plan_s();
+    if (!p) return fail(GFFTX_ERR_ALLOC, "host allocation failed");
+    p->device = device;
+    p->n = n;
+    p->batch = batch;
+    p->vtype = vtype;
+    p->ttype = ttype;
+    p->place = place;
+    p->flags = flags;
+    p->prec = (vtype == GFFTX_DOUBLE || vtype == GFFTX_COMPLEX_DOUBLE) ? PREC_F64 : PREC_F32;
+    p->real = real;
+    p->inverse = ttype == GFFTX_IDFT || ttype == GFFTX_IRDFT;
+    p->dist = n + ((real && (flags & GFFTX_FLAG_R2C_BINS)) ? 1 : 0);
+    int rc = build_c2c_passes(p);
+    if (rc == GFFTX_SUCCESS && real) rc = upload_split_table(p);
+    if (rc == GFFTX_SUCCESS && p->passes.size() > 1) {
+        const size_t bytes = (size_t)p->dist * batch * elem_bytes(p->prec);
+        rc = dev_alloc(p, &p->scratch[0], bytes);
+        if (rc == GFFTX_SUCCESS && p->passes.size() > 2) rc = dev_alloc(p, &p->scratch[1], bytes);
+    }
+    if (rc != GFFTX_SUCCESS) {
+        for (void* a : p->allocs) (void)hipFree(a);
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return GFFTX_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gfftx_plan_create(gfftx_plan* plan, uint64_t n, int vtype, int ttype, int place, int device, uint32_t flags)
+{
+    return create_plan(plan, n, 1, vtype, ttype, place, device, flags);
+}
+
+int gfftx_plan_create_batched(gfftx_plan* plan, uint64_t n, uint64_t batch, int vtype, int ttype, int place,
+                              int device, uint32_t flags)
+{
+    return create_plan(plan, n, batch, vtype, ttype, place, device, flags);
+}
+
+int gfftx_exec_inplace(gfftx_plan plan, void* data, void* stream)
+{
+    if (!plan) return fail(GFFTX_ERR_INVALID_ARG, "plan is NULL");
+    if (!data) return fail(GFFTX_ERR_INVALID_ARG, "data is NULL");
+    if (plan->place != GFFTX_IN_PLACE) return fail(GFFTX_ERR_INVALID_ARG, "plan is out-of-place");
+    return exec_any(plan, data, data, (hipStream_t)stream);
+}
+
+int gfftx_exec_oop(gfftx_plan plan, const void* src, void* dst, void* stream)
+{
+    if (!plan) return fail(GFFTX_ERR_INVALID_ARG, "plan is NULL");
+    if (!src || !dst) return fail(GFFTX_ERR_INVALID_ARG, "src/dst is NULL");
+    if (plan->place != GFFTX_OUT_OF_PLACE) return fail(GFFTX_ERR_INVALID_ARG, "plan is in-place");
+    if (src == dst) return fail(GFFTX_ERR_ALIAS, "out-of-place transform with src == dst");
+    return exec_any(plan, src, dst, (hipStream_t)stream);
+}
+
+int gfftx_plan_profile(gfftx_plan plan, double* total_ms, uint64_t* count, const char** names, int max,
+                       int* nslots)
+{
+    if (!plan || !total_ms || !count || max < 0) return fail(GFFTX_ERR_INVALID_ARG, "bad arguments");
+    if (!(plan->flags & GFFTX_FLAG_PROFILE)) return fail(GFFTX_ERR_INVALID_ARG, "plan was not created with GFFTX_FLAG_PROFILE");
+    DeviceGuard g(plan->device);
+    const int n = (int)plan->slots.size();
+    if (nslots) *nslots = n;
+    for (int i = 0; i < n; ++i) {
+        gfftx_plan_s::Slot& sl = plan->slots[i];
+        double tot = 0;
+        for (auto& ev : sl.pending) {
+            HIP_TRY(hipEventSynchronize(ev.second));
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, ev.first, ev.second));
+            tot += ms;
+            sl.pool.push_back(ev);
+        }
+        if (i < max) {
+            total_ms[i] = tot;
+            count[i] = sl.pending.size();
+            if (names) names[i] = sl.name.c_str();
+        }
+        sl.pending.clear();
+    }
+    return GFFTX_SUCCESS;
+}
+
+int gfftx_plan_destroy(gfftx_plan plan)
+{
+    if (!plan) return GFFTX_SUCCESS;
+    DeviceGuard g(plan->device);
+    for (auto& sl : plan->slots) {
+        for (auto& ev : sl.pending) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
+        for (auto& ev : sl.pool) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
+    }
+    for (void* a : plan->allocs) (void)hipFree(a);
+    delete plan;
+    return GFFTX_SUCCESS;
+}
+
+int gfftx_plan_describe(gfftx_plan plan, char* buf, size_t len)
+{
+    if (!plan || !buf || len == 0) return fail(GFFTX_ERR_INVALID_ARG, "bad arguments");
+    snprintf(buf, len, "%s", plan->desc.c_str());
+    return plan->desc.size() < len ? GFFTX_SUCCESS : fail(GFFTX_ERR_INVALID_ARG, "buffer too small");
+}
+
+int gfftx_digit_reverse(const void* src, void* dst, uint64_t radix, uint32_t digits, int vtype, void* stream)
+{
+    if (!src || !dst || src == dst) return fail(GFFTX_ERR_INVALID_ARG, "src/dst NULL or aliased");
+    if (radix < 2 || digits == 0) return fail(GFFTX_ERR_INVALID_ARG, "radix >= 2 and digits >= 1 required");
+    long double nn = 1;
+    for (uint32_t i = 0; i < digits; ++i) nn *= (long double)radix;
+    if (nn > 9.0e15L) return fail(GFFTX_ERR_INVALID_ARG, "radix^digits too large");
+    if (vtype < 0 || vtype > 3) return fail(GFFTX_ERR_INVALID_ARG, "unknown value type %d", vtype);
+    if (!is_device_ptr(src) || !is_device_ptr(dst))
+        return fail(GFFTX_ERR_INVALID_ARG, "gfftx_digit_reverse takes device pointers");
+    const int prec = (vtype == GFFTX_DOUBLE || vtype == GFFTX_COMPLEX_DOUBLE) ? PREC_F64 : PREC_F32;
+    hipError_t e = launch_digit_reverse(prec, src, dst, (long long)radix, (int)digits, (long long)nn,
+                                        (hipStream_t)stream);
+    if (e != hipSuccess) return fail(GFFTX_ERR_LAUNCH, "digit_reverse: %s", hipGetErrorString(e));
+    return GFFTX_SUCCESS;
+}
+
+int gfftx_ragged_create(gfftx_ragged* plan, const uint64_t* lengths, uint64_t count, int vtype, int ttype,
+                        int device, uint32_t flags)
+{
+    (void)lengths; (void)count; (void)vtype; (void)ttype; (void)device; (void)flags;
+    if (plan) *plan = nullptr;
+    return fail(GFFTX_ERR_UNSUPPORTED, "ragged batches are not available yet");
+}
+
+int gfftx_ragged_exec(gfftx_ragged plan, void* data, void* stream)
+{
+    (void)plan; (void)data; (void)stream;
+    return fail(GFFTX_ERR_UNSUPPORTED, "ragged batches are not available yet");
+}
+
+int gfftx_ragged_destroy(gfftx_ragged plan)
+{
+    (void)plan;
+    return GFFTX_SUCCESS;
+}
+
+const char* gfftx_strerror(int status)
+{
+    switch (status) {
+        case GFFTX_SUCCESS: return "success";
+        case GFFTX_ERR_INVALID_ARG: return "invalid argument";
+        case GFFTX_ERR_UNSUPPORTED: return "Requested transform is not compiled! (unsupported length/type/place)";
+        case GFFTX_ERR_ALLOC: return "allocation failed";
+        case GFFTX_ERR_DEVICE: return "HIP device error";
+        case GFFTX_ERR_ALIAS: return "out-of-place transform with src == dst";
+        case GFFTX_ERR_LAUNCH: return "kernel launch failed";
+        default: return "unknown status";
+    }
+}
+
+const char* gfftx_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

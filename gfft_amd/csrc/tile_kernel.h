@@ -1,0 +1,256 @@
+// tile_kernel.h -- batched Stockham FFT of one length L held in LDS (gfx950).
+//
+// One workgroup transforms NB independent sequences of length L.  Each sequence is worked
+// on by TPT = WG/NB threads.  Stage s (radix R_s, Ns = R_0*..*R_{s-1}) is the self-sorting
+// (Stockham, decimation in time) step
+//     for j in [0, L/R):  k = j mod Ns
+//         v[r] = x[j + r*L/R] * W_{Ns*R}^{k*r}          r = 0..R-1
+//         y[(j/Ns)*Ns*R + k + r*Ns] = DFT_R(v)[r]
+// so the output is in natural order and no digit-reversal pass exists.  The first stage
+// reads global memory straight into registers and the last stage writes global memory
+// straight from registers when the load and store access modes agree; otherwise the last
+// stage goes through LDS once more so that the global store is coalesced.
+//
+// The same kernel runs every pass of a multi-pass transform of length N = P_0*P_1*..: pass i
+// is the Stockham step above with radix P_i applied to the whole array, i.e. a batch of
+// N/P_i "sequences" j whose element r sits at j + r*N/P_i, followed by the store rule with
+// Ns = P_0*..*P_{i-1}.  The pass twiddle W_{Ns*P_i}^{k*r} is produced in FP64 from two
+// small FP64 tables (W^m = lo[m mod 2^b] * hi[m >> b]).
+//
+// Access modes: "T" (transform-contiguous) means consecutive sequences are adjacent in
+// memory (its == 1 on load, Ns > 1 on store); "E" (element-contiguous) means each sequence
+// is contiguous.  Lanes are mapped so that consecutive lanes touch consecutive addresses.
+#pragma once
+#include "fft_common.h"
+
+namespace gfftx {
+
+struct TileArgs {
+    const void* in;
+    void* out;
+    long long ntrans;   // sequences per batch item
+    long long its;      // input: distance between sequences (elements)
+    long long ies;      // input: distance between elements of a sequence
+    long long ns_pass;  // store rule Ns (1 = each sequence written contiguously)
+    long long bdist;    // distance between batch items (grid.y), elements
+    const void* tw;     // stage twiddles, working precision, rows of R entries
+    const double2* ptw_lo;  // pass twiddles (null: none)
+    const double2* ptw_hi;
+    int ptw_shift;
+    int conj_in;
+    int conj_out;
+    int pad_;
+    double scale;
+};
+
+template <typename T_, int L_, int NB_, int WG_, bool LOADT_, bool STORET_, int... Rs>
+struct TileCfg {
+    using T = T_;
+    using V = V2<T_>;
+    static constexpr int L = L_, NB = NB_, WG = WG_;
+    static constexpr bool LOADT = LOADT_, STORET = STORET_;
+    static constexpr int NS = sizeof...(Rs);
+    static constexpr int R[NS] = {Rs...};
+    static constexpr int TPT = WG / NB;
+    static_assert(WG % NB == 0, "WG must be a multiple of NB");
+    static constexpr int ns(int s)
+    {
+        int n = 1;
+        for (int i = 0; i < s; ++i) n *= R[i];
+        return n;
+    }
+    static_assert(ns(NS) == L, "radices must multiply to L");
+    static constexpr int nbf(int s) { return (L / R[s] + TPT - 1) / TPT; }
+    static constexpr int twoff(int s)
+    {
+        int o = 0;
+        for (int i = 1; i < s; ++i) o += ns(i) * R[i];
+        return o;
+    }
+    static constexpr int twsize() { return twoff(NS); }
+    // LDS: one padded element every 16, odd distance between sequences (bank spread).
+    static constexpr int LP = L + L / 16;
+    static constexpr int TS = (LP % 2 == 0) ? LP + 1 : LP;
+    static constexpr bool DIRECT = (LOADT == STORET);
+    static constexpr bool NEED_LDS = (NS > 1) || !DIRECT;
+    static constexpr int LDS_ELEMS = NEED_LDS ? NB * TS : 1;
+    __device__ static constexpr int lidx(int t, int e) { return t * TS + e + (e >> 4); }
+};
+
+template <class C> struct Ctx {
+    using V = typename C::V;
+    V* lds;
+    const V* in;
+    V* out;
+    const V* tw;
+    int t, jt;
+    long long tg;
+    bool valid;
+    TileArgs a;
+};
+
+template <typename V> __device__ __forceinline__ V to_v(double2 w);
+template <> __device__ __forceinline__ float2 to_v<float2>(double2 w) { return float2{(float)w.x, (float)w.y}; }
+template <> __device__ __forceinline__ double2 to_v<double2>(double2 w) { return w; }
+
+__device__ __forceinline__ double2 dmul(double2 a, double2 b)
+{
+    return double2{__builtin_fma(a.x, b.x, -(a.y * b.y)), __builtin_fma(a.x, b.y, a.y * b.x)};
+}
+
+__device__ __forceinline__ double2 pass_root(const TileArgs& a, long long m)
+{
+    const double2 lo = a.ptw_lo[m & ((1LL << a.ptw_shift) - 1)];
+    const double2 hi = a.ptw_hi[m >> a.ptw_shift];
+    return dmul(lo, hi);
+}
+
+template <class C, int S> __device__ __forceinline__ void tile_stage(Ctx<C>& x)
+{
+    using V = typename C::V;
+    constexpr int R = C::R[S];
+    constexpr int NSS = C::ns(S);
+    constexpr int B = C::nbf(S);
+    constexpr int LR = C::L / R;
+    constexpr bool FIRST = (S == 0);
+    constexpr bool LAST = (S == C::NS - 1);
+    constexpr bool GUARD = (B * C::TPT > LR);
+    V v[B][R];
+
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int j = x.jt + b * C::TPT;
+        if (GUARD && j >= LR) continue;
+        if constexpr (FIRST) {
+            const TileArgs& a = x.a;
+            if (x.valid) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const long long e = j + r * LR;
+                    V u = x.in[x.tg * a.its + e * a.ies];
+                    if (a.conj_in) u.y = -u.y;
+                    v[b][r] = u;
+                }
+                if (a.ptw_lo != nullptr) {
+                    // W_M^{kp*e}, e = j + r*LR: base W^{kp*j} times step^r, step = W^{kp*LR}
+                    const long long kp = x.tg % a.ns_pass;
+                    if (kp != 0) {
+                        const double2 step = pass_root(a, kp * LR);
+                        double2 w = pass_root(a, kp * j);
+                        v[b][0] = cmul(v[b][0], to_v<V>(w));
+#pragma unroll
+                        for (int r = 1; r < R; ++r) {
+                            w = dmul(w, step);
+                            v[b][r] = cmul(v[b][r], to_v<V>(w));
+                        }
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[b][r] = V{0, 0};
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b][r] = x.lds[C::lidx(x.t, j + r * LR)];
+            const int k = j % NSS;
+            const V* row = x.tw + C::twoff(S) + k * R;
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], row[r]);
+        }
+        Dft<R>::run(v[b]);
+    }
+
+    if constexpr (!FIRST) __syncthreads();
+
+    if constexpr (LAST && C::DIRECT) {
+        if (x.valid) {
+            const TileArgs& a = x.a;
+            const long long ns = a.ns_pass;
+            const long long base = (x.tg / ns) * ns * C::L + (x.tg % ns);
+            const double sc = a.scale;
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const int j = x.jt + b * C::TPT;
+                if (GUARD && j >= LR) continue;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    V u = v[b][r];
+                    if (a.conj_out) u.y = -u.y;
+                    if (sc != 1.0) u = cscale(u, (decltype(u.x))sc);
+                    x.out[base + (long long)(j + r * LR) * ns] = u;
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int j = x.jt + b * C::TPT;
+            if (GUARD && j >= LR) continue;
+            const int k = j % NSS;
+            const int base = (j / NSS) * NSS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) x.lds[C::lidx(x.t, base + r * NSS)] = v[b][r];
+        }
+        __syncthreads();
+    }
+}
+
+template <class C, int S> __device__ __forceinline__ void stage_loop(Ctx<C>& x)
+{
+    if constexpr (S < C::NS) {
+        tile_stage<C, S>(x);
+        stage_loop<C, S + 1>(x);
+    }
+}
+
+template <class C> __global__ __launch_bounds__(C::WG) void tile_kernel(TileArgs a)
+{
+    using V = typename C::V;
+    __shared__ V lds[C::LDS_ELEMS];
+    const int tid = threadIdx.x;
+    Ctx<C> x;
+    x.lds = lds;
+    x.a = a;
+    if (C::LOADT) {
+        x.t = tid % C::NB;
+        x.jt = tid / C::NB;
+    } else {
+        x.t = tid / C::TPT;
+        x.jt = tid % C::TPT;
+    }
+    x.tg = (long long)blockIdx.x * C::NB + x.t;
+    x.valid = x.tg < a.ntrans;
+    x.in = reinterpret_cast<const V*>(a.in) + (long long)blockIdx.y * a.bdist;
+    x.out = reinterpret_cast<V*>(a.out) + (long long)blockIdx.y * a.bdist;
+    x.tw = reinterpret_cast<const V*>(a.tw);
+
+    stage_loop<C, 0>(x);
+
+    if constexpr (!C::DIRECT) {
+        // transposed store: the LDS holds natural-order outputs of the NB sequences
+        constexpr int PER = C::L / C::TPT;
+        const long long ns = a.ns_pass;
+        const double sc = a.scale;
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int lin = tid + i * C::WG;
+            int ts, q;
+            if (C::STORET) {
+                ts = lin % C::NB;
+                q = lin / C::NB;
+            } else {
+                ts = lin / C::L;
+                q = lin % C::L;
+            }
+            const long long tgs = (long long)blockIdx.x * C::NB + ts;
+            if (tgs < a.ntrans) {
+                V u = lds[C::lidx(ts, q)];
+                if (a.conj_out) u.y = -u.y;
+                if (sc != 1.0) u = cscale(u, (decltype(u.x))sc);
+                x.out[(tgs / ns) * ns * C::L + (tgs % ns) + (long long)q * ns] = u;
+            }
+        }
+    }
+}
+
+}  // namespace gfftx

@@ -1,0 +1,75 @@
+"""The C ABI library loads and exports every symbol include/gfftx.h declares; argument and
+device errors come back as status codes.  CPU only (no compute calls)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+from gfft_amd import api
+
+HEADER = os.path.join(ROOT, "include", "gfftx.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(gfftx_[a-z_]+)\s*\(", text)))
+
+
+def test_header_declares_expected_entry_points():
+    names = declared_functions()
+    assert set(names) == set(api.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    L = api.lib()
+    for name in declared_functions():
+        assert hasattr(L, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", api.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r" T (gfftx_\w+)", out))
+    assert set(declared_functions()) <= exported
+
+
+def test_strerror_texts():
+    L = api.lib()
+    assert L.gfftx_strerror(0) == b"success"
+    assert b"not compiled" in L.gfftx_strerror(api.ERR_UNSUPPORTED)
+    assert L.gfftx_strerror(999) == b"unknown status"
+
+
+def test_argument_errors_without_device():
+    L = api.lib()
+    h = ctypes.c_void_p()
+    # NULL plan pointer and bad IDs are rejected before any device work
+    assert L.gfftx_plan_create(None, 1024, 0, 0, 0, 0, 0) == api.ERR_INVALID_ARG
+    assert L.gfftx_plan_create(ctypes.byref(h), 0, 0, 0, 0, 0, 0) == api.ERR_INVALID_ARG
+    assert L.gfftx_plan_create(ctypes.byref(h), 1024, 9, 0, 0, 0, 0) == api.ERR_INVALID_ARG
+    assert L.gfftx_plan_create(ctypes.byref(h), 1024, 0, 7, 0, 0, 0) == api.ERR_INVALID_ARG
+    assert L.gfftx_plan_create(ctypes.byref(h), 1024, 0, 0, 5, 0, 0) == api.ERR_INVALID_ARG
+    assert L.gfftx_plan_create(ctypes.byref(h), 1024, 0, 0, 0, 0, 0x100) == api.ERR_INVALID_ARG
+    assert L.gfftx_exec_inplace(None, None, None) == api.ERR_INVALID_ARG
+    assert L.gfftx_plan_destroy(None) == api.SUCCESS
+    assert L.gfftx_digit_reverse(None, None, 2, 3, 0, None) == api.ERR_INVALID_ARG
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK),
+                    reason="a GPU is visible")
+def test_no_device_is_a_clean_error():
+    L = api.lib()
+    h = ctypes.c_void_p()
+    rc = L.gfftx_plan_create(ctypes.byref(h), 1024, 0, 0, 0, 0, 0)
+    assert rc == api.ERR_DEVICE
+    assert b"device" in L.gfftx_last_error()
+    with pytest.raises(api.GfftxError):
+        api.Plan(1024)
+
+
+def test_cpp_policy_header_compiles():
+    """The C++ policy layer (GFFT surface + HIP<Dev>) compiles as plain host C++."""
+    src = os.path.join(ROOT, "tests", "cpp", "gfft_check.cpp")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), src],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

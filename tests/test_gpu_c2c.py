@@ -1,0 +1,194 @@
+"""GPU parity of complex transforms through the C ABI against the CPU oracle.
+
+Bars: FP64 relL2 <= 1e-13, FP32 relL2 <= 2e-7 against the long-double oracle (FP64 result);
+output order natural; inverse scaled by 1/N as GFFT's Backward (src/gfftswap.h:355-362).
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from gfft_amd import api, inputs
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = {np.complex128: 1e-13, np.complex64: 2e-7}
+VT = {np.complex128: api.COMPLEX_DOUBLE, np.complex64: api.COMPLEX_FLOAT}
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    import torch
+
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def oracle_fwd(x):
+    return O.ref_fft(x.astype(np.complex128), 1)
+
+
+def oracle_inv(x):
+    n = x.shape[-1]
+    return O.ref_fft(x.astype(np.complex128), -1) / n
+
+
+POW2 = [16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192]
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n", POW2)
+def test_single_pass_forward_inverse(torch_cuda, dtype, n):
+    x = inputs.complex_uniform(n, 17 + n).astype(dtype)
+    ref_f, ref_i = oracle_fwd(x), oracle_inv(x)
+    print(f"case n={n} dtype={np.dtype(dtype).name}", flush=True)
+    with api.Plan(n, VT[dtype], api.DFT, api.IN_PLACE) as p:
+        d = dev(torch_cuda, x)
+        p.fft(d)
+        assert rel_l2(host(d), ref_f) <= TOL[dtype]
+    with api.Plan(n, VT[dtype], api.IDFT, api.OUT_OF_PLACE) as p:
+        s = dev(torch_cuda, x)
+        o = torch_cuda.empty_like(s)
+        p.fft(s, o)
+        assert rel_l2(host(o), ref_i) <= TOL[dtype]
+        assert np.array_equal(host(s), x)  # src untouched
+
+
+def test_golden_vectors(torch_cuda, golden):
+    for n in [16, 32, 64, 128, 256, 1024]:
+        x = golden[f"c2c_in_{n}"]
+        with api.Plan(n, api.COMPLEX_DOUBLE, api.DFT, api.OUT_OF_PLACE) as p:
+            s = dev(torch_cuda, x)
+            o = torch_cuda.empty_like(s)
+            p.fft(s, o)
+            assert rel_l2(host(o), golden[f"c2c_fwd_{n}"]) <= 1e-15 * np.log2(n) * 4
+        with api.Plan(n, api.COMPLEX_DOUBLE, api.IDFT, api.IN_PLACE) as p:
+            d = dev(torch_cuda, x)
+            p.fft(d)
+            assert rel_l2(host(d), golden[f"c2c_inv_{n}"]) <= 1e-15 * np.log2(n) * 4
+
+
+def test_c1_config_fp64_1024(torch_cuda):
+    """N=2^10 FP64 in place forward, U[-1,1) seed 17; against the oracle and the GFFT restatement."""
+    n = 1024
+    x = inputs.complex_uniform(n, 17)
+    with api.Plan(n, api.DOUBLE, api.DFT, api.IN_PLACE) as p:
+        d = dev(torch_cuda, x)
+        p.fft(d)
+        y = host(d)
+    assert rel_l2(y, oracle_fwd(x)) <= 1e-13
+    assert rel_l2(y, O.gfft_inplace(x, 1)) <= 1e-13
+    # output order: the largest bin of a pure tone lands where GFFT puts it
+    tone = np.exp(-2j * np.pi * ((37 * np.arange(n)) % n) / n)
+    with api.Plan(n, api.DOUBLE, api.DFT, api.IN_PLACE) as p:
+        d = dev(torch_cuda, tone)
+        p.fft(d)
+        assert int(np.argmax(np.abs(host(d)))) == n - 37
+        assert int(np.argmax(np.abs(O.gfft_inplace(tone, 1)))) == n - 37
+
+
+def test_seq_kat(torch_cuda):
+    n = 4096
+    m = np.arange(n)
+    x = (2 * m) + 1j * (2 * m + 1)
+    k = np.arange(1, n)
+    ref = np.empty(n, np.complex128)
+    ref[0] = (2 + 2j) * n * (n - 1) / 2 + 1j * n
+    sin_half = np.sin(np.pi * np.minimum(k, n - k) / n)
+    ref[1:] = (2 + 2j) * n * 1j * np.exp(1j * np.pi * k / n) / (2 * sin_half)
+    with api.Plan(n, api.DOUBLE, api.DFT, api.IN_PLACE) as p:
+        d = dev(torch_cuda, x)
+        p.fft(d)
+        assert rel_l2(host(d), ref) <= 1e-14
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n,batch", [(16, 1000), (256, 333), (1024, 64), (4096, 17)])
+def test_batched(torch_cuda, dtype, n, batch):
+    x = inputs.complex_normal(n * batch, 5).astype(dtype).reshape(batch, n)
+    ref = np.stack([oracle_fwd(r) for r in x])
+    with api.Plan(n, VT[dtype], api.DFT, api.IN_PLACE, batch=batch) as p:
+        d = dev(torch_cuda, x)
+        p.fft(d)
+        y = host(d)
+    for b in range(batch):
+        assert rel_l2(y[b], ref[b]) <= TOL[dtype]
+
+
+def test_host_pointers_are_synchronous(torch_cuda):
+    n = 2048
+    x = inputs.complex_uniform(n, 3)
+    ref = oracle_fwd(x)
+    with api.Plan(n, api.COMPLEX_DOUBLE, api.DFT, api.IN_PLACE) as p:
+        h = x.copy()
+        p.fft(h)  # numpy array: host pointer, staged and synchronous
+        assert rel_l2(h, ref) <= 1e-13
+    with api.Plan(n, api.COMPLEX_DOUBLE, api.DFT, api.OUT_OF_PLACE) as p:
+        out = np.zeros_like(x)
+        p.fft(x, out)
+        assert rel_l2(out, ref) <= 1e-13
+
+
+def test_out_of_place_alias_rejected(torch_cuda):
+    with api.Plan(256, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE) as p:
+        d = torch_cuda.zeros(256, dtype=torch_cuda.complex64, device="cuda")
+        with pytest.raises(api.GfftxError) as e:
+            p.fft(d, d)
+        assert e.value.code == api.ERR_ALIAS
+
+
+def test_unsupported_length_raises(torch_cuda):
+    with pytest.raises(api.UnsupportedTransform):
+        api.Plan(7 * 64, api.COMPLEX_DOUBLE)
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n", [2 ** 16, 2 ** 18, 2 ** 20])
+def test_multi_pass(torch_cuda, dtype, n):
+    x = inputs.complex_normal(n, 24).astype(dtype)
+    ref = oracle_fwd(x)
+    print(f"case multipass n={n} dtype={np.dtype(dtype).name}", flush=True)
+    with api.Plan(n, VT[dtype], api.DFT, api.IN_PLACE) as p:
+        d = dev(torch_cuda, x)
+        p.fft(d)
+        assert rel_l2(host(d), ref) <= TOL[dtype]
+    with api.Plan(n, VT[dtype], api.IDFT, api.OUT_OF_PLACE) as p:
+        s = dev(torch_cuda, ref.astype(dtype))
+        o = torch_cuda.empty_like(s)
+        p.fft(s, o)
+        assert rel_l2(host(o), x) <= 2 * TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_c2_config_2p24_round_trip(torch_cuda, dtype):
+    """N=2^24 forward then inverse in place, N(0,1) seed 24: forward against the FP64 oracle,
+    round trip against the input."""
+    n = 2 ** 24
+    x = inputs.complex_normal(n, 24).astype(dtype)
+    ref = oracle_fwd(x)
+    print(f"case 2^24 dtype={np.dtype(dtype).name}", flush=True)
+    with api.Plan(n, VT[dtype], api.DFT, api.IN_PLACE) as pf, \
+            api.Plan(n, VT[dtype], api.IDFT, api.IN_PLACE) as pi:
+        d = dev(torch_cuda, x)
+        pf.fft(d)
+        y = host(d)
+        e_fwd = rel_l2(y, ref)
+        pi.fft(d)
+        e_rt = rel_l2(host(d), x)
+    print(f"2^24 {np.dtype(dtype).name}: fwd relL2 {e_fwd:.3e} round-trip {e_rt:.3e}")
+    assert e_fwd <= TOL[dtype]
+    assert e_rt <= 2 * TOL[dtype]
+
+
+@pytest.mark.parametrize("radix,digits", [(2, 10), (2, 16), (3, 7), (5, 5), (7, 4)])
+def test_digit_reverse_bit_exact(torch_cuda, radix, digits):
+    """The permutation kernel equals GFFTswap2's in-place digit reversal bit for bit."""
+    n = radix ** digits
+    x = inputs.complex_normal(n, radix * 100 + digits)
+    s = dev(torch_cuda, x)
+    o = torch_cuda.empty_like(s)
+    api.digit_reverse(s, o, radix, digits, api.COMPLEX_DOUBLE)
+    assert np.array_equal(host(o), O.swap(x, radix, digits))

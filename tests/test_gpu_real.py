@@ -1,0 +1,103 @@
+"""GPU parity of real transforms (RDFT/IRDFT) through the C ABI.
+
+GFFT's RDFT maps 2n reals to n complex values in NR packing [X_0, X_n, X_1 .. X_{n-1}]
+(src/gfftswap.h:239-287); IRDFT inverts it including the 1/n of Backward.  The bins layout
+(n+1 complex bins) is the engine's extra format for the 3*2^20 workload."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from gfft_amd import api, inputs
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+VT = {np.complex128: api.DOUBLE, np.complex64: api.FLOAT}
+TOL = {np.complex128: 1e-13, np.complex64: 2e-7}
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    import torch
+
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def packed_ref(z):
+    """Exact NR-packed RDFT of the 2n reals interleaved in z (long-double oracle)."""
+    n = z.size
+    r = np.stack([z.real, z.imag], 1).ravel().astype(np.float64)
+    X = O.ref_fft(r.astype(np.complex128), 1)[: n + 1]
+    out = X[:n].copy()
+    out[0] = X[0].real + 1j * X[n].real
+    return out, X
+
+
+def test_golden_rdft(torch_cuda, golden):
+    for n in [2, 4, 8, 16, 64, 256]:
+        z = golden[f"r2c_in_{n}"]
+        with api.Plan(n, api.DOUBLE, api.RDFT, api.IN_PLACE) as p:
+            d = dev(torch_cuda, z)
+            p.fft(d)
+            assert rel_l2(host(d), golden[f"r2c_fwd_{n}"]) <= 1e-14
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n", [32, 512, 4096, 3 * 256, 5 * 128, 2 ** 16])
+def test_rdft_irdft(torch_cuda, dtype, n):
+    z = inputs.complex_normal(n, 40 + n).astype(dtype)
+    ref, _ = packed_ref(z)
+    print(f"case real n={n} dtype={np.dtype(dtype).name}", flush=True)
+    try:
+        pf = api.Plan(n, VT[dtype], api.RDFT, api.OUT_OF_PLACE)
+    except api.UnsupportedTransform:
+        pytest.skip(f"no kernel for n={n}")
+    with pf, api.Plan(n, VT[dtype], api.IRDFT, api.OUT_OF_PLACE) as pi:
+        s = dev(torch_cuda, z)
+        o = torch_cuda.empty_like(s)
+        pf.fft(s, o)
+        y = host(o)
+        assert rel_l2(y, ref) <= TOL[dtype]
+        if n <= 4096 and dtype == np.complex128 and (n & (n - 1)) == 0:
+            assert rel_l2(y, O.gfft_real_inplace(z, 1)) <= 1e-13
+        back = torch_cuda.empty_like(s)
+        pi.fft(o, back)
+        assert rel_l2(host(back), z) <= 2 * TOL[dtype]
+        assert np.array_equal(host(s), z)
+
+
+def test_rdft_bins_layout_and_odd_rejected(torch_cuda):
+    n = 1024
+    z = inputs.complex_normal(n, 9)
+    _, X = packed_ref(z)
+    with api.Plan(n, api.DOUBLE, api.RDFT, api.OUT_OF_PLACE, flags=api.FLAG_R2C_BINS) as p:
+        s = dev(torch_cuda, z)
+        o = torch_cuda.zeros(n + 1, dtype=torch_cuda.complex128, device="cuda")
+        p.fft(s, o)
+        assert rel_l2(host(o), X) <= 1e-13
+    with pytest.raises(api.UnsupportedTransform):
+        api.Plan(27, api.DOUBLE, api.RDFT, api.IN_PLACE)
+
+
+def test_c3_config_r2c_3x2p20(torch_cuda):
+    """N = 3*2^20 real FP32: 8 integer-bin cosines + N(0, 0.01^2) noise, seed 3; N/2+1 bins."""
+    nr = 3 * 2 ** 20
+    x, freqs, amps = inputs.real_sinusoids(nr, 3)
+    n = nr // 2
+    z = (x[0::2] + 1j * x[1::2]).astype(np.complex64)
+    ref = O.ref_fft(np.stack([z.real, z.imag], 1).ravel().astype(np.complex128), 1)[: n + 1]
+    with api.Plan(n, api.FLOAT, api.RDFT, api.OUT_OF_PLACE, flags=api.FLAG_R2C_BINS) as p:
+        s = dev(torch_cuda, z)
+        o = torch_cuda.zeros(n + 1, dtype=torch_cuda.complex64, device="cuda")
+        p.fft(s, o)
+        y = host(o)
+    err = rel_l2(y, ref)
+    print(f"C3 relL2 {err:.3e}")
+    assert err <= 2e-7
+    # peaks at the seeded bins with amplitude A*N/2
+    for f, a in zip(freqs, amps):
+        assert abs(abs(y[f]) - a * nr / 2) < 1e-3 * a * nr / 2 + 50
