@@ -23,8 +23,13 @@ template <class C> static hipError_t launch_tile(const TileArgs& a, dim3 grid, h
      &launch_tile<GFFTX_CFG(TYPE, L, NB, WG, LT, ST, __VA_ARGS__)>,                           \
      "tile_" #TYPE "_L" #L "_nb" #NB "_wg" #WG "_" #LT #ST}
 
-// Batched single-pass (E->E) tiles: TPT = L/16 threads per sequence, 256-thread workgroups.
-#define GFFTX_BATCH_SET(PREC, TYPE)                                          \
+// Batched single-pass (E->E) tiles.  TPT = L/16 threads per sequence (one radix-16
+// butterfly each), NB sequences per workgroup with NB*TPT close to 256.  Radix-2/4/8/3/5
+// stages spread their butterflies over the same TPT threads.
+#define GFFTX_BATCH_POW2(PREC, TYPE)                                         \
+    GFFTX_TILE(PREC, TYPE, 2, 64, 64, false, false, 2),                     \
+    GFFTX_TILE(PREC, TYPE, 4, 64, 64, false, false, 4),                     \
+    GFFTX_TILE(PREC, TYPE, 8, 64, 64, false, false, 8),                     \
     GFFTX_TILE(PREC, TYPE, 16, 64, 64, false, false, 16),                   \
     GFFTX_TILE(PREC, TYPE, 32, 64, 128, false, false, 16, 2),               \
     GFFTX_TILE(PREC, TYPE, 64, 64, 256, false, false, 16, 4),               \
@@ -36,23 +41,58 @@ template <class C> static hipError_t launch_tile(const TileArgs& a, dim3 grid, h
     GFFTX_TILE(PREC, TYPE, 4096, 1, 256, false, false, 16, 16, 16),         \
     GFFTX_TILE(PREC, TYPE, 8192, 1, 512, false, false, 16, 16, 16, 2)
 
+#define GFFTX_BATCH_MIXED(PREC, TYPE)                                        \
+    GFFTX_TILE(PREC, TYPE, 48, 85, 255, false, false, 16, 3),               \
+    GFFTX_TILE(PREC, TYPE, 80, 51, 255, false, false, 16, 5),               \
+    GFFTX_TILE(PREC, TYPE, 96, 42, 252, false, false, 16, 2, 3),            \
+    GFFTX_TILE(PREC, TYPE, 160, 25, 250, false, false, 16, 2, 5),           \
+    GFFTX_TILE(PREC, TYPE, 192, 21, 252, false, false, 16, 4, 3),           \
+    GFFTX_TILE(PREC, TYPE, 320, 12, 240, false, false, 16, 4, 5),           \
+    GFFTX_TILE(PREC, TYPE, 384, 10, 240, false, false, 16, 8, 3),           \
+    GFFTX_TILE(PREC, TYPE, 640, 6, 240, false, false, 16, 8, 5),            \
+    GFFTX_TILE(PREC, TYPE, 768, 5, 240, false, false, 16, 16, 3),           \
+    GFFTX_TILE(PREC, TYPE, 1280, 3, 240, false, false, 16, 16, 5),          \
+    GFFTX_TILE(PREC, TYPE, 1536, 2, 192, false, false, 16, 16, 2, 3),       \
+    GFFTX_TILE(PREC, TYPE, 2560, 2, 320, false, false, 16, 16, 2, 5),       \
+    GFFTX_TILE(PREC, TYPE, 3072, 2, 384, false, false, 16, 16, 4, 3),       \
+    GFFTX_TILE(PREC, TYPE, 5120, 1, 320, false, false, 16, 16, 4, 5),       \
+    GFFTX_TILE(PREC, TYPE, 6144, 1, 384, false, false, 16, 16, 8, 3)
+
 // Pass tiles for multi-pass transforms: first pass (T->E) and later passes (T->T).
-#define GFFTX_PASS_SET(PREC, TYPE, NB4096, WG4096, NB1024, WG1024)                          \
+#define GFFTX_PASS_SET(PREC, TYPE, NB4096, WG4096, NB1024, WG1024, NB1536, WG1536)         \
     GFFTX_TILE(PREC, TYPE, 256, 16, 256, true, false, 16, 16),                             \
     GFFTX_TILE(PREC, TYPE, 256, 16, 256, true, true, 16, 16),                              \
     GFFTX_TILE(PREC, TYPE, 1024, NB1024, WG1024, true, false, 16, 16, 4),                  \
     GFFTX_TILE(PREC, TYPE, 1024, NB1024, WG1024, true, true, 16, 16, 4),                   \
+    GFFTX_TILE(PREC, TYPE, 1536, NB1536, WG1536, true, false, 16, 16, 2, 3),               \
+    GFFTX_TILE(PREC, TYPE, 1536, NB1536, WG1536, true, true, 16, 16, 2, 3),                \
     GFFTX_TILE(PREC, TYPE, 4096, NB4096, WG4096, true, false, 16, 16, 16),                 \
     GFFTX_TILE(PREC, TYPE, 4096, NB4096, WG4096, true, true, 16, 16, 16)
 
 // Host table.  Not const: a const namespace-scope table would be emitted into the device
 // image as well; both compilation passes still instantiate every kernel listed here.
 TileEntry kTiles[] = {
-    GFFTX_BATCH_SET(PREC_F32, float),
+    GFFTX_BATCH_POW2(PREC_F32, float),
+    GFFTX_BATCH_MIXED(PREC_F32, float),
+    GFFTX_TILE(PREC_F32, float, 10240, 1, 640, false, false, 16, 16, 8, 5),
+    GFFTX_TILE(PREC_F32, float, 12288, 1, 768, false, false, 16, 16, 16, 3),
     GFFTX_TILE(PREC_F32, float, 16384, 1, 1024, false, false, 16, 16, 16, 4),
-    GFFTX_PASS_SET(PREC_F32, float, 4, 1024, 8, 512),
-    GFFTX_BATCH_SET(PREC_F64, double),
-    GFFTX_PASS_SET(PREC_F64, double, 2, 512, 4, 256),
+    GFFTX_PASS_SET(PREC_F32, float, 4, 1024, 8, 512, 8, 768),
+    GFFTX_TILE(PREC_F32, float, 4096, 2, 512, true, false, 16, 16, 16),
+    GFFTX_TILE(PREC_F32, float, 4096, 2, 512, true, true, 16, 16, 16),
+    GFFTX_TILE(PREC_F32, float, 4096, 1, 256, true, false, 16, 16, 16),
+    GFFTX_TILE(PREC_F32, float, 4096, 1, 256, true, true, 16, 16, 16),
+    GFFTX_TILE(PREC_F32, float, 256, 32, 512, true, false, 16, 16),
+    GFFTX_TILE(PREC_F32, float, 256, 32, 512, true, true, 16, 16),
+    GFFTX_TILE(PREC_F32, float, 16, 64, 64, true, true, 16),
+    GFFTX_TILE(PREC_F32, float, 64, 64, 256, true, true, 16, 4),
+    GFFTX_BATCH_POW2(PREC_F64, double),
+    GFFTX_BATCH_MIXED(PREC_F64, double),
+    GFFTX_PASS_SET(PREC_F64, double, 2, 512, 4, 256, 4, 384),
+    GFFTX_TILE(PREC_F64, double, 4096, 1, 256, true, false, 16, 16, 16),
+    GFFTX_TILE(PREC_F64, double, 4096, 1, 256, true, true, 16, 16, 16),
+    GFFTX_TILE(PREC_F64, double, 16, 64, 64, true, true, 16),
+    GFFTX_TILE(PREC_F64, double, 64, 64, 256, true, true, 16, 4),
 };
 const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 

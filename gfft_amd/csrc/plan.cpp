@@ -92,25 +92,29 @@ struct Pass {
     double2* ptw_hi = nullptr;
     int ptw_shift = 0;
     long long ptw_m = 0;
+    int xcd = 0;
 };
 
 size_t elem_bytes(int prec) { return prec == PREC_F64 ? 16 : 8; }
 
-const TileEntry* find_tile(int prec, int L, bool loadT, bool storeT)
+// First registered tile for (prec, L, modes), or the one with nb == want_nb when present.
+const TileEntry* find_tile(int prec, int L, bool loadT, bool storeT, int want_nb = 0)
 {
+    const TileEntry* first = nullptr;
     for (int i = 0; i < kNumTiles; ++i) {
         const TileEntry& e = kTiles[i];
-        if (e.prec == prec && e.L == L && e.loadT == loadT && e.storeT == storeT) return &e;
+        if (e.prec == prec && e.L == L && e.loadT == loadT && e.storeT == storeT) {
+            if (want_nb && e.nb == want_nb) return &e;
+            if (!first) first = &e;
+        }
     }
-    return nullptr;
+    return first;
 }
 
-int max_single_tile(int prec)
+int env_int(const char* name, int dflt)
 {
-    int best = 0;
-    for (int i = 0; i < kNumTiles; ++i)
-        if (kTiles[i].prec == prec && !kTiles[i].loadT && !kTiles[i].storeT) best = std::max(best, kTiles[i].L);
-    return best;
+    const char* v = getenv(name);
+    return v ? atoi(v) : dflt;
 }
 
 bool only_235(unsigned long long n)
@@ -118,6 +122,18 @@ bool only_235(unsigned long long n)
     for (unsigned long long p : {2ull, 3ull, 5ull})
         while (n % p == 0) n /= p;
     return n == 1;
+}
+
+// Bandwidth (TB/s) a multi-pass tile of length L reaches on MI355X with XCD remapping,
+// measured over N = 2^24 passes; lengths not listed are interpolated by size class.
+double pass_bandwidth(int prec, int L)
+{
+    const bool f32 = prec == PREC_F32;
+    if (L <= 16) return f32 ? 5.5 : 5.6;
+    if (L <= 64) return f32 ? 4.6 : 4.7;
+    if (L <= 256) return f32 ? 4.2 : 4.6;
+    if (L <= 1536) return f32 ? 3.1 : 3.5;
+    return f32 ? 2.6 : 2.5;
 }
 
 // Split n into pass lengths available as T-mode tiles (largest first).  An override list
@@ -150,38 +166,34 @@ bool choose_passes(int prec, unsigned long long n, std::vector<int>& out)
         if (kTiles[i].prec == prec && kTiles[i].loadT && kTiles[i].storeT) lens.push_back(kTiles[i].L);
     std::sort(lens.begin(), lens.end());
     lens.erase(std::unique(lens.begin(), lens.end()), lens.end());
-    // prefer the fewest passes, then the most balanced split
-    for (int npass = 2; npass <= 4; ++npass) {
-        std::vector<int> best;
-        double best_score = 1e300;
-        std::vector<int> cur(npass, 0);
-        // enumerate non-increasing sequences of lens with product n
-        std::function<void(int, unsigned long long, int)> rec = [&](int pos, unsigned long long rem, int maxi) {
-            if (pos == npass) {
-                if (rem == 1) {
-                    double mx = 0, mn = 1e300;
-                    for (int v : cur) { mx = std::max(mx, (double)v); mn = std::min(mn, (double)v); }
-                    const double score = mx / mn;
-                    if (score < best_score) { best_score = score; best = cur; }
-                }
-                return;
+    // Enumerate non-increasing sequences of pass lengths with product n (2..4 passes) and
+    // keep the one with the lowest modelled time: each pass moves 2 n b_c bytes at the
+    // bandwidth its tile reaches (pass_bandwidth()).
+    std::vector<int> best, cur;
+    double best_cost = 1e300;
+    std::function<void(unsigned long long, int)> rec = [&](unsigned long long rem, int maxi) {
+        if (rem == 1) {
+            if (cur.size() < 2 || !find_tile(prec, cur[0], true, false)) return;
+            double cost = 0;
+            for (int v : cur) cost += 1.0 / pass_bandwidth(prec, v);
+            if (cost < best_cost - 1e-12) {
+                best_cost = cost;
+                best = cur;
             }
-            for (int i = maxi; i >= 0; --i) {
-                if (rem % (unsigned long long)lens[i]) continue;
-                cur[pos] = lens[i];
-                rec(pos + 1, rem / lens[i], i);
-            }
-        };
-        rec(0, n, (int)lens.size() - 1);
-        if (!best.empty()) {
-            bool ok = find_tile(prec, best[0], true, false) != nullptr;
-            if (ok) {
-                out = best;
-                return true;
-            }
+            return;
         }
-    }
-    return false;
+        if (cur.size() == 4) return;
+        for (int i = maxi; i >= 0; --i) {
+            if (rem % (unsigned long long)lens[i]) continue;
+            cur.push_back(lens[i]);
+            rec(rem / lens[i], i);
+            cur.pop_back();
+        }
+    };
+    rec(n, (int)lens.size() - 1);
+    if (best.empty()) return false;
+    out = best;
+    return true;
 }
 
 }  // namespace
@@ -300,13 +312,12 @@ int upload_pass_table(gfftx_plan_s* p, Pass& ps, long long M)
 int build_c2c_passes(gfftx_plan_s* p)
 {
     const unsigned long long n = p->n;
-    const int maxL = max_single_tile(p->prec);
     char buf[256];
     p->desc = "{\"n\":" + std::to_string(n) + ",\"batch\":" + std::to_string(p->batch) +
               ",\"precision\":\"" + (p->prec == PREC_F64 ? "f64" : "f32") + "\",\"passes\":[";
-    if (n <= (unsigned long long)maxL) {
-        const TileEntry* k = find_tile(p->prec, (int)n, false, false);
-        if (!k) return fail(GFFTX_ERR_UNSUPPORTED, "no single-pass kernel for n=%llu", n);
+    const TileEntry* single = n <= (1ull << 30) ? find_tile(p->prec, (int)n, false, false) : nullptr;
+    if (single) {
+        const TileEntry* k = single;
         Pass ps;
         ps.k = k;
         ps.ntrans = (long long)p->batch;
@@ -327,10 +338,11 @@ int build_c2c_passes(gfftx_plan_s* p)
         long long ns = 1;
         for (size_t i = 0; i < lens.size(); ++i) {
             const int L = lens[i];
-            const TileEntry* k = find_tile(p->prec, L, true, i > 0);
+            const TileEntry* k = find_tile(p->prec, L, true, i > 0, env_int("GFFTX_TNB", 0));
             if (!k) return fail(GFFTX_ERR_UNSUPPORTED, "no pass kernel L=%d", L);
             Pass ps;
             ps.k = k;
+            ps.xcd = env_int("GFFTX_XCD", 1);
             ps.ntrans = (long long)(n / L);
             ps.its = 1;
             ps.ies = (long long)(n / L);
@@ -344,8 +356,8 @@ int build_c2c_passes(gfftx_plan_s* p)
                 if (rc) return rc;
             }
             p->passes.push_back(ps);
-            snprintf(buf, sizeof(buf), "%s{\"kernel\":\"%s\",\"L\":%d,\"ns\":%lld,\"ntrans\":%lld}",
-                     i ? "," : "", k->name, L, ns, ps.ntrans);
+            snprintf(buf, sizeof(buf), "%s{\"kernel\":\"%s\",\"L\":%d,\"ns\":%lld,\"ntrans\":%lld,\"xcd\":%d}",
+                     i ? "," : "", k->name, L, ns, ps.ntrans, ps.xcd);
             p->desc += buf;
             ns *= L;
         }
@@ -472,6 +484,7 @@ int run_passes(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
         a.conj_in = (p->inverse && i == 0) ? 1 : 0;
         a.conj_out = (p->inverse && i == m - 1) ? 1 : 0;
         a.scale = (p->inverse && i == m - 1) ? 1.0 / (double)p->n : 1.0;
+        a.xcd = ps.xcd;
         const long long nb = ps.k->nb;
         dim3 grid((unsigned)((ps.ntrans + nb - 1) / nb), ps.batch);
         std::pair<hipEvent_t, hipEvent_t> ev;
@@ -582,7 +595,8 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
     if (ttype < 0 || ttype > 3) return fail(GFFTX_ERR_INVALID_ARG, "unknown transform type %d", ttype);
     if (place != GFFTX_IN_PLACE && place != GFFTX_OUT_OF_PLACE)
         return fail(GFFTX_ERR_INVALID_ARG, "unknown place %d", place);
-    if (flags & ~(GFFTX_FLAG_R2C_BINS | GFFTX_FLAG_SPLIT_MFMA | GFFTX_FLAG_NO_AUTOTUNE | GFFTX_FLAG_GRAPH))
+    if (flags & ~(GFFTX_FLAG_R2C_BINS | GFFTX_FLAG_SPLIT_MFMA | GFFTX_FLAG_NO_AUTOTUNE | GFFTX_FLAG_GRAPH |
+                  GFFTX_FLAG_PROFILE))
         return fail(GFFTX_ERR_INVALID_ARG, "unknown flags 0x%x", flags);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {

@@ -39,9 +39,20 @@ struct TileArgs {
     int ptw_shift;
     int conj_in;
     int conj_out;
-    int pad_;
+    int xcd;            // 1: remap workgroups so neighbouring tiles share one XCD's L2
     double scale;
 };
+
+// Workgroups are dispatched round-robin over the 8 XCDs, so consecutive blockIdx values land
+// on different L2 caches.  With xcd remapping, tile (b%8)*(full/8) + b/8 is processed by
+// block b: the tiles one XCD receives are consecutive, and T-mode tiles that share cache
+// lines are fetched into the same L2.  The tail (nbx % 8 blocks) keeps the identity map.
+__device__ __forceinline__ unsigned tile_index(unsigned b, unsigned nbx, int remap)
+{
+    if (!remap) return b;
+    const unsigned full = nbx - (nbx & 7u);
+    return b < full ? (b & 7u) * (full >> 3) + (b >> 3) : b;
+}
 
 template <typename T_, int L_, int NB_, int WG_, bool LOADT_, bool STORET_, int... Rs>
 struct TileCfg {
@@ -218,7 +229,8 @@ template <class C> __global__ __launch_bounds__(C::WG) void tile_kernel(TileArgs
         x.t = tid / C::TPT;
         x.jt = tid % C::TPT;
     }
-    x.tg = (long long)blockIdx.x * C::NB + x.t;
+    const unsigned tile = tile_index(blockIdx.x, gridDim.x, a.xcd);
+    x.tg = (long long)tile * C::NB + x.t;
     x.valid = x.tg < a.ntrans;
     x.in = reinterpret_cast<const V*>(a.in) + (long long)blockIdx.y * a.bdist;
     x.out = reinterpret_cast<V*>(a.out) + (long long)blockIdx.y * a.bdist;
@@ -242,7 +254,7 @@ template <class C> __global__ __launch_bounds__(C::WG) void tile_kernel(TileArgs
                 ts = lin / C::L;
                 q = lin % C::L;
             }
-            const long long tgs = (long long)blockIdx.x * C::NB + ts;
+            const long long tgs = (long long)tile * C::NB + ts;
             if (tgs < a.ntrans) {
                 V u = lds[C::lidx(ts, q)];
                 if (a.conj_out) u.y = -u.y;

@@ -57,8 +57,50 @@ def test_single_pass_forward_inverse(torch_cuda, dtype, n):
         assert np.array_equal(host(s), x)  # src untouched
 
 
+MIXED = [2, 4, 8, 48, 80, 96, 160, 192, 320, 384, 640, 768, 1280, 1536, 2560, 3072, 5120, 6144]
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n", MIXED)
+def test_small_and_mixed_radix(torch_cuda, dtype, n):
+    batch = 7
+    x = inputs.complex_uniform(n * batch, 300 + n).astype(dtype).reshape(batch, n)
+    ref = np.stack([oracle_fwd(r) for r in x])
+    with api.Plan(n, VT[dtype], api.DFT, api.IN_PLACE, batch=batch) as p:
+        d = dev(torch_cuda, x)
+        p.fft(d)
+        y = host(d)
+    for b in range(batch):
+        assert rel_l2(y[b], ref[b]) <= TOL[dtype]
+    with api.Plan(n, VT[dtype], api.IDFT, api.OUT_OF_PLACE) as p:
+        s = dev(torch_cuda, x[0])
+        o = torch_cuda.empty_like(s)
+        p.fft(s, o)
+        assert rel_l2(host(o), oracle_inv(x[0])) <= TOL[dtype]
+
+
+@pytest.mark.parametrize("n", [10240, 12288, 16384])
+def test_large_single_pass_fp32(torch_cuda, n):
+    x = inputs.complex_normal(n, n).astype(np.complex64)
+    with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.IN_PLACE) as p:
+        d = dev(torch_cuda, x)
+        p.fft(d)
+        assert rel_l2(host(d), oracle_fwd(x)) <= 2e-7
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n", [3 * 2 ** 19, 1536 * 256])
+def test_mixed_radix_multi_pass(torch_cuda, dtype, n):
+    x = inputs.complex_normal(n, 3).astype(dtype)
+    with api.Plan(n, VT[dtype], api.DFT, api.OUT_OF_PLACE) as p:
+        s = dev(torch_cuda, x)
+        o = torch_cuda.empty_like(s)
+        p.fft(s, o)
+        assert rel_l2(host(o), oracle_fwd(x)) <= TOL[dtype]
+
+
 def test_golden_vectors(torch_cuda, golden):
-    for n in [16, 32, 64, 128, 256, 1024]:
+    for n in [2, 4, 8, 16, 32, 64, 128, 256, 768, 1024]:
         x = golden[f"c2c_in_{n}"]
         with api.Plan(n, api.COMPLEX_DOUBLE, api.DFT, api.OUT_OF_PLACE) as p:
             s = dev(torch_cuda, x)
