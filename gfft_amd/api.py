@@ -34,7 +34,8 @@ SUCCESS, ERR_INVALID_ARG, ERR_UNSUPPORTED, ERR_ALLOC, ERR_DEVICE, ERR_ALIAS, ERR
 EXPORTED = [
     "gfftx_plan_create", "gfftx_plan_create_batched", "gfftx_exec_inplace", "gfftx_exec_oop",
     "gfftx_plan_destroy", "gfftx_plan_describe", "gfftx_plan_profile", "gfftx_ragged_create", "gfftx_ragged_exec",
-    "gfftx_ragged_destroy", "gfftx_digit_reverse", "gfftx_strerror", "gfftx_last_error",
+    "gfftx_ragged_destroy", "gfftx_ragged_describe", "gfftx_ragged_profile", "gfftx_digit_reverse",
+    "gfftx_strerror", "gfftx_last_error",
 ]
 
 _lib = None
@@ -73,6 +74,8 @@ def lib():
         L.gfftx_ragged_create.argtypes = [pp, ctypes.POINTER(u64), u64, ci, ci, ci, u32]
         L.gfftx_ragged_exec.argtypes = [vp, vp, vp]
         L.gfftx_ragged_destroy.argtypes = [vp]
+        L.gfftx_ragged_describe.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
+        L.gfftx_ragged_profile.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64)]
         L.gfftx_digit_reverse.argtypes = [vp, vp, u64, u32, ci, vp]
         L.gfftx_strerror.argtypes = [ci]
         L.gfftx_strerror.restype = ctypes.c_char_p
@@ -190,6 +193,18 @@ class RaggedPlan:
     def fft(self, data, stream=None):
         _check(lib().gfftx_ragged_exec(self._h, _ptr(data), _stream(data, stream)), "ragged_exec")
         return data
+
+    def describe(self) -> str:
+        buf = ctypes.create_string_buffer(1 << 16)
+        _check(lib().gfftx_ragged_describe(self._h, buf, len(buf)), "ragged_describe")
+        return buf.value.decode()
+
+    def profile(self):
+        """(total ms, launches) since the last call (needs FLAG_PROFILE)."""
+        tot = ctypes.c_double()
+        cnt = ctypes.c_uint64()
+        _check(lib().gfftx_ragged_profile(self._h, ctypes.byref(tot), ctypes.byref(cnt)), "ragged_profile")
+        return tot.value, int(cnt.value)
 
     def close(self):
         if getattr(self, "_h", None):

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "../../include/gfftx.h"
+#include "ragged.h"
 #include "registry.h"
 
 using namespace gfftx;
@@ -238,13 +239,13 @@ int dev_alloc(gfftx_plan_s* p, void** out, size_t bytes)
     return GFFTX_SUCCESS;
 }
 
-template <typename T> void pack_stage_table(const TileEntry* k, std::vector<T>& out)
+template <typename T> void pack_stage_table(int nrad, const int* radices, std::vector<T>& out)
 {
     // stage s >= 1: rows k in [0, Ns), entries r in [0, R): W_{Ns*R}^{k*r}
     out.clear();
-    long long ns = k->radices[0];
-    for (int s = 1; s < k->nrad; ++s) {
-        const long long R = k->radices[s];
+    long long ns = radices[0];
+    for (int s = 1; s < nrad; ++s) {
+        const long long R = radices[s];
         for (long long kk = 0; kk < ns; ++kk)
             for (long long r = 0; r < R; ++r) {
                 long double c, si;
@@ -256,30 +257,49 @@ template <typename T> void pack_stage_table(const TileEntry* k, std::vector<T>& 
     }
 }
 
-int upload_stage_table(gfftx_plan_s* p, const TileEntry* k, void** out)
+// Device allocation recorded in an owner's list (plans and ragged plans).
+int dev_alloc_list(std::vector<void*>& owned, void** out, size_t bytes)
+{
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(out, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(GFFTX_ERR_ALLOC, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    owned.push_back(*out);
+    return GFFTX_SUCCESS;
+}
+
+int upload_stage_table_r(std::vector<void*>& owned, int prec, int nrad, const int* radices, void** out)
 {
     *out = nullptr;
-    if (k->nrad <= 1) return GFFTX_SUCCESS;
-    if (p->prec == PREC_F64) {
+    if (nrad <= 1) return GFFTX_SUCCESS;
+    if (prec == PREC_F64) {
         std::vector<double> t;
-        pack_stage_table(k, t);
-        int rc = dev_alloc(p, out, t.size() * sizeof(double));
+        pack_stage_table(nrad, radices, t);
+        int rc = dev_alloc_list(owned, out, t.size() * sizeof(double));
         if (rc) return rc;
         HIP_TRY(hipMemcpy(*out, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
     } else {
         std::vector<float> t;
-        pack_stage_table(k, t);
-        int rc = dev_alloc(p, out, t.size() * sizeof(float));
+        pack_stage_table(nrad, radices, t);
+        int rc = dev_alloc_list(owned, out, t.size() * sizeof(float));
         if (rc) return rc;
         HIP_TRY(hipMemcpy(*out, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
     }
     return GFFTX_SUCCESS;
 }
 
-int upload_pass_table(gfftx_plan_s* p, Pass& ps, long long M)
+int upload_stage_table(gfftx_plan_s* p, const TileEntry* k, void** out)
+{
+    return upload_stage_table_r(p->allocs, p->prec, k->nrad, k->radices, out);
+}
+
+// FP64 two-level table for W_M^m: lo[m mod 2^b], hi[m >> b], 2^b ~ sqrt(M).
+int upload_root_tables(std::vector<void*>& owned, long long M, double2** lo_out, double2** hi_out, int* shift)
 {
     int b = 0;
-    while ((1LL << (2 * b)) < M) ++b;  // lo table 2^b ~ sqrt(M)
+    while ((1LL << (2 * b)) < M) ++b;
     const long long nlo = 1LL << b, nhi = (M + nlo - 1) / nlo + 1;
     std::vector<double> lo(2 * nlo), hi(2 * nhi);
     for (long long m = 0; m < nlo; ++m) {
@@ -296,17 +316,22 @@ int upload_pass_table(gfftx_plan_s* p, Pass& ps, long long M)
     }
     void* dlo = nullptr;
     void* dhi = nullptr;
-    int rc = dev_alloc(p, &dlo, lo.size() * sizeof(double));
+    int rc = dev_alloc_list(owned, &dlo, lo.size() * sizeof(double));
     if (rc) return rc;
-    rc = dev_alloc(p, &dhi, hi.size() * sizeof(double));
+    rc = dev_alloc_list(owned, &dhi, hi.size() * sizeof(double));
     if (rc) return rc;
     HIP_TRY(hipMemcpy(dlo, lo.data(), lo.size() * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(dhi, hi.data(), hi.size() * sizeof(double), hipMemcpyHostToDevice));
-    ps.ptw_lo = (double2*)dlo;
-    ps.ptw_hi = (double2*)dhi;
-    ps.ptw_shift = b;
-    ps.ptw_m = M;
+    *lo_out = (double2*)dlo;
+    *hi_out = (double2*)dhi;
+    *shift = b;
     return GFFTX_SUCCESS;
+}
+
+int upload_pass_table(gfftx_plan_s* p, Pass& ps, long long M)
+{
+    ps.ptw_m = M;
+    return upload_root_tables(p->allocs, M, &ps.ptw_lo, &ps.ptw_hi, &ps.ptw_shift);
 }
 
 int build_c2c_passes(gfftx_plan_s* p)
@@ -647,6 +672,221 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
 
 }  // namespace
 
+
+// ---------------------------------------------------------------------------------------
+// Ragged batches: one persistent launch over a list of work items (ragged.hip).
+struct gfftx_ragged_s {
+    int device = 0;
+    bool inverse = false;
+    uint32_t flags = 0;
+    unsigned long long total = 0;          // complex elements of the whole batch
+    std::vector<void*> allocs;
+    RaggedArgs args{};
+    int grid = 0;
+    std::string desc;
+    // hipGraph replay (GFFTX_FLAG_GRAPH)
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    void* graph_data = nullptr;
+    // host staging
+    void* stage = nullptr;
+    // profiling (GFFTX_FLAG_PROFILE): events around the launch
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending, pool;
+};
+
+namespace {
+
+int ragged_single_cfg(long long L)
+{
+    for (int id = 0; id < ragged_num_cfgs(); ++id) {
+        const RaggedCfgInfo c = ragged_cfg_info(id);
+        if (!c.loadT && !c.storeT && c.L == L) return id;
+    }
+    return -1;
+}
+
+int ragged_pass1_cfg(long long P)
+{
+    for (int id = 0; id < ragged_num_cfgs(); ++id) {
+        const RaggedCfgInfo c = ragged_cfg_info(id);
+        if (!c.loadT && c.storeT && c.L == P) return id;
+    }
+    return -1;
+}
+
+int ragged_build(gfftx_ragged_s* r, const uint64_t* lengths, uint64_t count)
+{
+    // classes by length, in order of first appearance
+    std::vector<long long> lens;
+    std::vector<std::vector<long long>> members;  // element offsets per class
+    std::vector<long long> offsets(count);
+    long long off = 0;
+    for (uint64_t i = 0; i < count; ++i) {
+        const long long L = (long long)lengths[i];
+        offsets[i] = off;
+        off += L;
+        size_t c = 0;
+        while (c < lens.size() && lens[c] != L) ++c;
+        if (c == lens.size()) {
+            lens.push_back(L);
+            members.emplace_back();
+        }
+        members[c].push_back(offsets[i]);
+    }
+    r->total = (unsigned long long)off;
+    std::vector<RaggedClass> classes(lens.size());
+    struct HostItem {
+        RaggedItem it;
+        double cost;
+    };
+    std::vector<HostItem> items;
+    long long scratch = 0;
+    char buf[160];
+    r->desc = "{\"transforms\":" + std::to_string(count) + ",\"points\":" + std::to_string(off) + ",\"classes\":[";
+    for (size_t c = 0; c < lens.size(); ++c) {
+        const long long L = lens[c];
+        RaggedClass& rc = classes[c];
+        memset(&rc, 0, sizeof(rc));
+        rc.L = L;
+        rc.count = (long long)members[c].size();
+        const int single = ragged_single_cfg(L);
+        const int p1 = (L % 64 == 0) ? ragged_pass1_cfg(L / 64) : -1;
+        const double unit = (double)L * std::log2((double)L);
+        if (single >= 0) {
+            const RaggedCfgInfo ci = ragged_cfg_info(single);
+            void* offs = nullptr;
+            int e = dev_alloc_list(r->allocs, &offs, members[c].size() * sizeof(long long));
+            if (e) return e;
+            HIP_TRY(hipMemcpy(offs, members[c].data(), members[c].size() * sizeof(long long), hipMemcpyHostToDevice));
+            rc.seq_off = (const long long*)offs;
+            void* tw = nullptr;
+            e = upload_stage_table_r(r->allocs, PREC_F32, ci.nrad, ci.radices, &tw);
+            if (e) return e;
+            rc.tw0 = tw;
+            const long long ntiles = (rc.count + ci.nb - 1) / ci.nb;
+            for (long long t = 0; t < ntiles; ++t) {
+                HostItem h;
+                memset(&h.it, 0, sizeof(h.it));
+                h.it.cfg = single;
+                h.it.cfg1 = -1;
+                h.it.cls = (int)c;
+                h.it.tile = (int)t;
+                h.cost = unit * (double)std::min<long long>(ci.nb, rc.count - t * ci.nb);
+                items.push_back(h);
+            }
+            snprintf(buf, sizeof(buf), "%s{\"L\":%lld,\"count\":%lld,\"cfg\":%d,\"nb\":%d}", c ? "," : "", L,
+                     rc.count, single, ci.nb);
+        } else if (p1 >= 0) {
+            const RaggedCfgInfo c0 = ragged_cfg_info(RAGGED_CFG_T64), c1 = ragged_cfg_info(p1);
+            void* tw = nullptr;
+            int e = upload_stage_table_r(r->allocs, PREC_F32, c0.nrad, c0.radices, &tw);
+            if (e) return e;
+            rc.tw0 = tw;
+            e = upload_stage_table_r(r->allocs, PREC_F32, c1.nrad, c1.radices, &tw);
+            if (e) return e;
+            rc.tw1 = tw;
+            double2 *lo, *hi;
+            e = upload_root_tables(r->allocs, L, &lo, &hi, &rc.ptw_shift);
+            if (e) return e;
+            rc.ptw_lo = lo;
+            rc.ptw_hi = hi;
+            for (long long o : members[c]) {
+                HostItem h;
+                memset(&h.it, 0, sizeof(h.it));
+                h.it.cfg = RAGGED_CFG_T64;
+                h.it.cfg1 = p1;
+                h.it.cls = (int)c;
+                h.it.ntiles0 = (int)((L / 64 + c0.nb - 1) / c0.nb);
+                h.it.ntiles1 = (64 + c1.nb - 1) / c1.nb;
+                h.it.off = o;
+                h.it.soff = scratch;
+                scratch += L;
+                h.cost = unit * 1.3;  // two passes through L2 on one CU
+                items.push_back(h);
+            }
+            snprintf(buf, sizeof(buf), "%s{\"L\":%lld,\"count\":%lld,\"cfg\":[%d,%d]}", c ? "," : "", L, rc.count,
+                     RAGGED_CFG_T64, p1);
+        } else {
+            return fail(GFFTX_ERR_UNSUPPORTED, "ragged batch: no kernel for length %lld", L);
+        }
+        r->desc += buf;
+    }
+    // longest processing time first
+    std::stable_sort(items.begin(), items.end(), [](const HostItem& x, const HostItem& y) { return x.cost > y.cost; });
+    std::vector<RaggedItem> host_items(items.size());
+    for (size_t i = 0; i < items.size(); ++i) host_items[i] = items[i].it;
+    void *d_items = nullptr, *d_classes = nullptr, *d_counter = nullptr, *d_scratch = nullptr;
+    int e = dev_alloc_list(r->allocs, &d_items, host_items.size() * sizeof(RaggedItem));
+    if (!e) e = dev_alloc_list(r->allocs, &d_classes, classes.size() * sizeof(RaggedClass));
+    if (!e) e = dev_alloc_list(r->allocs, &d_counter, sizeof(unsigned long long));
+    if (!e) e = dev_alloc_list(r->allocs, &d_scratch, (size_t)std::max<long long>(scratch, 1) * sizeof(float2));
+    if (e) return e;
+    HIP_TRY(hipMemcpy(d_items, host_items.data(), host_items.size() * sizeof(RaggedItem), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_classes, classes.data(), classes.size() * sizeof(RaggedClass), hipMemcpyHostToDevice));
+    r->args.items = (const RaggedItem*)d_items;
+    r->args.classes = (const RaggedClass*)d_classes;
+    r->args.counter = (unsigned long long*)d_counter;
+    r->args.scratch = (float2*)d_scratch;
+    r->args.nitems = (int)host_items.size();
+    r->args.inverse = r->inverse ? 1 : 0;
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, r->device));
+    const int per_cu = env_int("GFFTX_RAGGED_WG_PER_CU", 3);
+    r->grid = std::max(1, std::min<int>((int)host_items.size(), prop.multiProcessorCount * per_cu));
+    r->desc += "],\"items\":" + std::to_string(host_items.size()) + ",\"grid\":" + std::to_string(r->grid) +
+               ",\"scratch_points\":" + std::to_string(scratch) + "}";
+    return GFFTX_SUCCESS;
+}
+
+int ragged_launch(gfftx_ragged_s* r, void* data, hipStream_t s)
+{
+    RaggedArgs a = r->args;
+    a.data = (float2*)data;
+    std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+    const bool prof = (r->flags & GFFTX_FLAG_PROFILE) != 0;
+    if (prof) {
+        if (r->pool.empty()) {
+            hipEvent_t x, y;
+            HIP_TRY(hipEventCreate(&x));
+            HIP_TRY(hipEventCreate(&y));
+            r->pool.push_back({x, y});
+        }
+        ev = r->pool.back();
+        r->pool.pop_back();
+        HIP_TRY(hipEventRecord(ev.first, s));
+    }
+    if (r->flags & GFFTX_FLAG_GRAPH) {
+        if (!r->graph_exec || r->graph_data != data) {
+            if (r->graph_exec) (void)hipGraphExecDestroy(r->graph_exec);
+            if (r->graph) (void)hipGraphDestroy(r->graph);
+            r->graph_exec = nullptr;
+            r->graph = nullptr;
+            if (!r->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&r->cap_stream, hipStreamNonBlocking));
+            HIP_TRY(hipStreamBeginCapture(r->cap_stream, hipStreamCaptureModeThreadLocal));
+            hipError_t le = launch_ragged(a, r->grid, r->cap_stream);
+            hipGraph_t g = nullptr;
+            hipError_t ce = hipStreamEndCapture(r->cap_stream, &g);
+            if (le != hipSuccess) return fail(GFFTX_ERR_LAUNCH, "ragged capture: %s", hipGetErrorString(le));
+            if (ce != hipSuccess) return fail(GFFTX_ERR_DEVICE, "end capture: %s", hipGetErrorString(ce));
+            r->graph = g;
+            HIP_TRY(hipGraphInstantiate(&r->graph_exec, g, nullptr, nullptr, 0));
+            r->graph_data = data;
+        }
+        HIP_TRY(hipGraphLaunch(r->graph_exec, s));
+    } else {
+        hipError_t le = launch_ragged(a, r->grid, s);
+        if (le != hipSuccess) return fail(GFFTX_ERR_LAUNCH, "ragged_kernel: %s", hipGetErrorString(le));
+    }
+    if (prof) {
+        HIP_TRY(hipEventRecord(ev.second, s));
+        r->pending.push_back(ev);
+    }
+    return GFFTX_SUCCESS;
+}
+
+}  // namespace
+
 extern "C" {
 
 int gfftx_plan_create(gfftx_plan* plan, uint64_t n, int vtype, int ttype, int place, int device, uint32_t flags)
@@ -742,23 +982,103 @@ int gfftx_digit_reverse(const void* src, void* dst, uint64_t radix, uint32_t dig
     return GFFTX_SUCCESS;
 }
 
+
 int gfftx_ragged_create(gfftx_ragged* plan, const uint64_t* lengths, uint64_t count, int vtype, int ttype,
                         int device, uint32_t flags)
 {
-    (void)lengths; (void)count; (void)vtype; (void)ttype; (void)device; (void)flags;
-    if (plan) *plan = nullptr;
-    return fail(GFFTX_ERR_UNSUPPORTED, "ragged batches are not available yet");
+    if (!plan) return fail(GFFTX_ERR_INVALID_ARG, "plan pointer is NULL");
+    *plan = nullptr;
+    if (!lengths || count == 0) return fail(GFFTX_ERR_INVALID_ARG, "empty ragged batch");
+    if (count > (1ull << 24)) return fail(GFFTX_ERR_INVALID_ARG, "too many transforms");
+    if (vtype != GFFTX_FLOAT && vtype != GFFTX_COMPLEX_FLOAT)
+        return fail(GFFTX_ERR_UNSUPPORTED, "ragged batches are FP32 (FLOAT / COMPLEX_FLOAT)");
+    if (ttype != GFFTX_DFT && ttype != GFFTX_IDFT)
+        return fail(GFFTX_ERR_UNSUPPORTED, "ragged batches are complex transforms (DFT / IDFT)");
+    if (flags & ~(GFFTX_FLAG_GRAPH | GFFTX_FLAG_PROFILE | GFFTX_FLAG_NO_AUTOTUNE))
+        return fail(GFFTX_ERR_INVALID_ARG, "unsupported flags 0x%x for a ragged batch", flags);
+    for (uint64_t i = 0; i < count; ++i)
+        if (lengths[i] == 0) return fail(GFFTX_ERR_INVALID_ARG, "zero length at %llu", (unsigned long long)i);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        return fail(GFFTX_ERR_DEVICE, "no HIP device available");
+    }
+    if (device < 0 || device >= ndev) return fail(GFFTX_ERR_DEVICE, "device %d out of range (%d)", device, ndev);
+    DeviceGuard g(device);
+    if (!g.ok) return fail(GFFTX_ERR_DEVICE, "cannot select device %d", device);
+    gfftx_ragged_s* r = new (std::nothrow) gfftx_ragged_s();
+    if (!r) return fail(GFFTX_ERR_ALLOC, "host allocation failed");
+    r->device = device;
+    r->inverse = ttype == GFFTX_IDFT;
+    r->flags = flags;
+    int rc = ragged_build(r, lengths, count);
+    if (rc != GFFTX_SUCCESS) {
+        for (void* a : r->allocs) (void)hipFree(a);
+        delete r;
+        return rc;
+    }
+    *plan = r;
+    return GFFTX_SUCCESS;
 }
 
 int gfftx_ragged_exec(gfftx_ragged plan, void* data, void* stream)
 {
-    (void)plan; (void)data; (void)stream;
-    return fail(GFFTX_ERR_UNSUPPORTED, "ragged batches are not available yet");
+    if (!plan) return fail(GFFTX_ERR_INVALID_ARG, "plan is NULL");
+    if (!data) return fail(GFFTX_ERR_INVALID_ARG, "data is NULL");
+    DeviceGuard g(plan->device);
+    if (!g.ok) return fail(GFFTX_ERR_DEVICE, "cannot select device %d", plan->device);
+    hipStream_t s = (hipStream_t)stream;
+    if (is_device_ptr(data)) return ragged_launch(plan, data, s);
+    const size_t bytes = (size_t)plan->total * sizeof(float2);
+    if (!plan->stage) {
+        int rc = dev_alloc_list(plan->allocs, &plan->stage, bytes);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipMemcpyAsync(plan->stage, data, bytes, hipMemcpyHostToDevice, s));
+    int rc = ragged_launch(plan, plan->stage, s);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(data, plan->stage, bytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return GFFTX_SUCCESS;
+}
+
+int gfftx_ragged_describe(gfftx_ragged plan, char* buf, size_t len)
+{
+    if (!plan || !buf || len == 0) return fail(GFFTX_ERR_INVALID_ARG, "bad arguments");
+    snprintf(buf, len, "%s", plan->desc.c_str());
+    return plan->desc.size() < len ? GFFTX_SUCCESS : fail(GFFTX_ERR_INVALID_ARG, "buffer too small");
+}
+
+int gfftx_ragged_profile(gfftx_ragged plan, double* total_ms, uint64_t* count)
+{
+    if (!plan || !total_ms || !count) return fail(GFFTX_ERR_INVALID_ARG, "bad arguments");
+    if (!(plan->flags & GFFTX_FLAG_PROFILE)) return fail(GFFTX_ERR_INVALID_ARG, "plan was not created with GFFTX_FLAG_PROFILE");
+    DeviceGuard g(plan->device);
+    double tot = 0;
+    for (auto& ev : plan->pending) {
+        HIP_TRY(hipEventSynchronize(ev.second));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, ev.first, ev.second));
+        tot += ms;
+        plan->pool.push_back(ev);
+    }
+    *total_ms = tot;
+    *count = plan->pending.size();
+    plan->pending.clear();
+    return GFFTX_SUCCESS;
 }
 
 int gfftx_ragged_destroy(gfftx_ragged plan)
 {
-    (void)plan;
+    if (!plan) return GFFTX_SUCCESS;
+    DeviceGuard g(plan->device);
+    if (plan->graph_exec) (void)hipGraphExecDestroy(plan->graph_exec);
+    if (plan->graph) (void)hipGraphDestroy(plan->graph);
+    if (plan->cap_stream) (void)hipStreamDestroy(plan->cap_stream);
+    for (auto& ev : plan->pending) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
+    for (auto& ev : plan->pool) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
+    for (void* a : plan->allocs) (void)hipFree(a);
+    delete plan;
     return GFFTX_SUCCESS;
 }
 

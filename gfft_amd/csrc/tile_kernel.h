@@ -41,6 +41,7 @@ struct TileArgs {
     int conj_out;
     int xcd;            // 1: remap workgroups so neighbouring tiles share one XCD's L2
     double scale;
+    const long long* seq_off;  // optional: element offset of every sequence (E-mode in/out)
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs, so consecutive blockIdx values land
@@ -96,7 +97,9 @@ template <class C> struct Ctx {
     const V* tw;
     int t, jt;
     long long tg;
-    bool valid;
+    long long base_in;   // element offset of this thread's sequence (input side)
+    bool valid;          // sequence exists
+    bool active;         // thread takes part (threads beyond C::WG idle through the barriers)
     TileArgs a;
 };
 
@@ -131,14 +134,14 @@ template <class C, int S> __device__ __forceinline__ void tile_stage(Ctx<C>& x)
 #pragma unroll
     for (int b = 0; b < B; ++b) {
         const int j = x.jt + b * C::TPT;
-        if (GUARD && j >= LR) continue;
+        if (!x.active || (GUARD && j >= LR)) continue;
         if constexpr (FIRST) {
             const TileArgs& a = x.a;
             if (x.valid) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const long long e = j + r * LR;
-                    V u = x.in[x.tg * a.its + e * a.ies];
+                    V u = x.in[x.base_in + e * a.ies];
                     if (a.conj_in) u.y = -u.y;
                     v[b][r] = u;
                 }
@@ -174,10 +177,10 @@ template <class C, int S> __device__ __forceinline__ void tile_stage(Ctx<C>& x)
     if constexpr (!FIRST) __syncthreads();
 
     if constexpr (LAST && C::DIRECT) {
-        if (x.valid) {
+        if (x.valid && x.active) {
             const TileArgs& a = x.a;
             const long long ns = a.ns_pass;
-            const long long base = (x.tg / ns) * ns * C::L + (x.tg % ns);
+            const long long base = a.seq_off ? a.seq_off[x.tg] : (x.tg / ns) * ns * C::L + (x.tg % ns);
             const double sc = a.scale;
 #pragma unroll
             for (int b = 0; b < B; ++b) {
@@ -196,7 +199,7 @@ template <class C, int S> __device__ __forceinline__ void tile_stage(Ctx<C>& x)
 #pragma unroll
         for (int b = 0; b < B; ++b) {
             const int j = x.jt + b * C::TPT;
-            if (GUARD && j >= LR) continue;
+            if (!x.active || (GUARD && j >= LR)) continue;
             const int k = j % NSS;
             const int base = (j / NSS) * NSS * R + k;
 #pragma unroll
@@ -214,14 +217,18 @@ template <class C, int S> __device__ __forceinline__ void stage_loop(Ctx<C>& x)
     }
 }
 
-template <class C> __global__ __launch_bounds__(C::WG) void tile_kernel(TileArgs a)
+// One tile (NB sequences) of the pass described by `a`; `tile` selects the sequences,
+// `by` the batch item.  Callable from any kernel whose workgroup has at least C::WG threads
+// and which provides C::LDS_ELEMS elements of LDS; extra threads only join the barriers.
+template <class C>
+__device__ __forceinline__ void tile_body(const TileArgs& a, unsigned tile, unsigned by, typename C::V* lds)
 {
     using V = typename C::V;
-    __shared__ V lds[C::LDS_ELEMS];
     const int tid = threadIdx.x;
     Ctx<C> x;
     x.lds = lds;
     x.a = a;
+    x.active = tid < C::WG;
     if (C::LOADT) {
         x.t = tid % C::NB;
         x.jt = tid / C::NB;
@@ -229,12 +236,16 @@ template <class C> __global__ __launch_bounds__(C::WG) void tile_kernel(TileArgs
         x.t = tid / C::TPT;
         x.jt = tid % C::TPT;
     }
-    const unsigned tile = tile_index(blockIdx.x, gridDim.x, a.xcd);
+    if (!x.active) {
+        x.t = 0;
+        x.jt = 0;
+    }
     x.tg = (long long)tile * C::NB + x.t;
     x.valid = x.tg < a.ntrans;
-    x.in = reinterpret_cast<const V*>(a.in) + (long long)blockIdx.y * a.bdist;
-    x.out = reinterpret_cast<V*>(a.out) + (long long)blockIdx.y * a.bdist;
+    x.in = reinterpret_cast<const V*>(a.in) + (long long)by * a.bdist;
+    x.out = reinterpret_cast<V*>(a.out) + (long long)by * a.bdist;
     x.tw = reinterpret_cast<const V*>(a.tw);
+    x.base_in = x.valid ? (a.seq_off ? a.seq_off[x.tg] : x.tg * a.its) : 0;
 
     stage_loop<C, 0>(x);
 
@@ -243,26 +254,36 @@ template <class C> __global__ __launch_bounds__(C::WG) void tile_kernel(TileArgs
         constexpr int PER = C::L / C::TPT;
         const long long ns = a.ns_pass;
         const double sc = a.scale;
+        if (x.active) {
 #pragma unroll
-        for (int i = 0; i < PER; ++i) {
-            const int lin = tid + i * C::WG;
-            int ts, q;
-            if (C::STORET) {
-                ts = lin % C::NB;
-                q = lin / C::NB;
-            } else {
-                ts = lin / C::L;
-                q = lin % C::L;
-            }
-            const long long tgs = (long long)tile * C::NB + ts;
-            if (tgs < a.ntrans) {
-                V u = lds[C::lidx(ts, q)];
-                if (a.conj_out) u.y = -u.y;
-                if (sc != 1.0) u = cscale(u, (decltype(u.x))sc);
-                x.out[(tgs / ns) * ns * C::L + (tgs % ns) + (long long)q * ns] = u;
+            for (int i = 0; i < PER; ++i) {
+                const int lin = tid + i * C::WG;
+                int ts, q;
+                if (C::STORET) {
+                    ts = lin % C::NB;
+                    q = lin / C::NB;
+                } else {
+                    ts = lin / C::L;
+                    q = lin % C::L;
+                }
+                const long long tgs = (long long)tile * C::NB + ts;
+                if (tgs < a.ntrans) {
+                    V u = lds[C::lidx(ts, q)];
+                    if (a.conj_out) u.y = -u.y;
+                    if (sc != 1.0) u = cscale(u, (decltype(u.x))sc);
+                    const long long ob = a.seq_off ? a.seq_off[tgs] : (tgs / ns) * ns * C::L + (tgs % ns);
+                    x.out[ob + (long long)q * ns] = u;
+                }
             }
         }
     }
+}
+
+template <class C> __global__ __launch_bounds__(C::WG) void tile_kernel(TileArgs a)
+{
+    using V = typename C::V;
+    __shared__ V lds[C::LDS_ELEMS];
+    tile_body<C>(a, tile_index(blockIdx.x, gridDim.x, a.xcd), blockIdx.y, lds);
 }
 
 }  // namespace gfftx

@@ -114,6 +114,11 @@ int gfftx_ragged_create(gfftx_ragged* plan, const uint64_t* lengths, uint64_t co
                         int ttype, int device, uint32_t flags);
 int gfftx_ragged_exec(gfftx_ragged plan, void* data, void* stream);
 int gfftx_ragged_destroy(gfftx_ragged plan);
+/* JSON description of a ragged plan (classes, items, grid).  No reference counterpart. */
+int gfftx_ragged_describe(gfftx_ragged plan, char* buf, size_t len);
+/* Summed duration and count of ragged launches since the last query (GFFTX_FLAG_PROFILE).
+ * No reference counterpart. */
+int gfftx_ragged_profile(gfftx_ragged plan, double* total_ms, uint64_t* count);
 
 /* Digit-reversal permutation dst[rev(k)] = src[k] over radix^digits complex elements
  * (device pointers).  Replaces GFFTswap2<radix,digits,T>::apply (src/gfftswap.h:74-140) when

@@ -1,0 +1,103 @@
+"""GPU parity of the ragged batch (many lengths, one persistent launch) through the C ABI.
+
+Every transform is compared with the long-double oracle at relL2 <= 2e-7 (FP32)."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from gfft_amd import api, inputs
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+C4_LENGTHS = sorted({m << k for m in (1, 3, 5) for k in range(0, 17) if 256 <= (m << k) <= 65536})
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    import torch
+
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def check_all(y, x, lengths, inverse=False, tol=2e-7):
+    off = 0
+    worst = 0.0
+    for L in lengths:
+        xs = x[off:off + L].astype(np.complex128)
+        ref = O.ref_fft(xs, -1) / L if inverse else O.ref_fft(xs, 1)
+        e = rel_l2(y[off:off + L], ref)
+        worst = max(worst, e)
+        assert e <= tol, (L, e)
+        off += L
+    return worst
+
+
+def test_all_lengths_once(torch_cuda):
+    lengths = [16, 32, 48, 64, 80, 96, 128, 160, 192] + C4_LENGTHS + C4_LENGTHS[::-1]
+    assert len(C4_LENGTHS) == 25
+    x = inputs.complex_normal(sum(lengths), 77).astype(np.complex64)
+    p = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT)
+    d = dev(torch_cuda, x)
+    p.fft(d)
+    worst = check_all(host(d), x, lengths)
+    print("ragged all lengths worst relL2", worst, p.describe()[:200])
+    p.close()
+
+
+def test_inverse_round_trip(torch_cuda):
+    lengths = [256, 5120, 65536, 384, 640, 12288]
+    x = inputs.complex_normal(sum(lengths), 78).astype(np.complex64)
+    pf = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT)
+    pi = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.IDFT)
+    d = dev(torch_cuda, x)
+    pf.fft(d)
+    y = host(d)
+    pi.fft(d)
+    back = host(d)
+    check_all(y, x, lengths)
+    assert rel_l2(back, x) <= 4e-7
+    xi = y.copy()
+    d2 = dev(torch_cuda, xi)
+    pi.fft(d2)
+    check_all(host(d2), xi, lengths, inverse=True)
+
+
+def test_unsupported_lengths(torch_cuda):
+    with pytest.raises(api.UnsupportedTransform):
+        api.RaggedPlan([256, 7 * 64], api.COMPLEX_FLOAT)
+    with pytest.raises(api.UnsupportedTransform):
+        api.RaggedPlan([256], api.COMPLEX_DOUBLE)
+
+
+def test_graph_replays_match_eager(torch_cuda):
+    lengths = list(inputs.ragged_lengths(300, 11))
+    x = inputs.complex_normal(int(sum(lengths)), 12).astype(np.complex64)
+    eager = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT)
+    graph = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT, flags=api.FLAG_GRAPH)
+    d = dev(torch_cuda, x)
+    eager.fft(d)
+    ref = host(d)
+    g = dev(torch_cuda, x)
+    for rep in range(4):
+        g.copy_(torch_cuda.from_numpy(x))
+        graph.fft(g)
+        torch_cuda.cuda.synchronize()
+        _ = (g.abs().sum()).item()  # eager work between replays
+        assert np.array_equal(host(g), ref), f"replay {rep} differs from the eager result"
+
+
+def test_c4_config_4096_transforms(torch_cuda):
+    """4096 FP32 transforms, lengths log-uniform over {2^k,3*2^k,5*2^k} in [2^8,2^16], seed 4096."""
+    lengths = [int(v) for v in inputs.ragged_lengths(4096, 4096)]
+    total = sum(lengths)
+    x = inputs.complex_normal(total, 4096).astype(np.complex64)
+    p = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT)
+    d = dev(torch_cuda, x)
+    p.fft(d)
+    worst = check_all(host(d), x, lengths)
+    print(f"C4: {len(lengths)} transforms, {total} points, worst relL2 {worst:.3e}")
