@@ -96,6 +96,18 @@ TileEntry kTiles[] = {
 };
 const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
+template <class C> static hipError_t launch_mfma(const MfmaArgs& a, dim3 grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(mfma_fft_kernel<C>, grid, dim3(C::WG), 0, s, a);
+    return hipGetLastError();
+}
+
+MfmaEntry kMfma[] = {
+    {256, 16, &launch_mfma<MfmaCfg<256, 16>>, "mfma_split_f32_L256_nb16"},
+    {4096, 1, &launch_mfma<MfmaCfg<4096, 1>>, "mfma_split_f32_L4096_nb1"},
+};
+const int kNumMfma = sizeof(kMfma) / sizeof(kMfma[0]);
+
 // ---------------------------------------------------------------------------------------
 // Real-transform split pass.  Forward (post-pass of RDFT) for k in [1, n/2]:
 //   h1 = (Z_k + conj Z_{n-k})/2,  h2 = -i (Z_k - conj Z_{n-k})/2,  w = W_{2n}^k

@@ -1,0 +1,195 @@
+// mfma_kernel.h -- FP32 FFT of length 16^S whose radix-16 stages run on the matrix cores.
+//
+// Stage s of the Stockham sweep computes, for each of the L/16 columns j (k = j mod Ns),
+//     Y[q][j] = sum_r F16[q][r] * (x[j + r L/16] * W_{16 Ns}^{k r}),
+// i.e. a 16x16 complex DFT matrix times a 16 x (L/16) complex matrix.  In real form the DFT
+// matrix is 32x32; with the K index interleaved as (Re v_r, Im v_r) = k = 2r, 2r+1:
+//     rows of Re Y:  A0[q][2r] =  Re F[q][r],  A0[q][2r+1] = -Im F[q][r]
+//     rows of Im Y:  A1[q][2r] =  Im F[q][r],  A1[q][2r+1] =  Re F[q][r].
+// Each 16-column block is two v_mfma_f32_16x16x32_f16 products (A0 and A1 against one
+// B operand).  FP32 accuracy is recovered with a hi/lo split of both operands,
+//     A B ~= Ah Bh + Ah Bl + Al Bh        (FP32 accumulation, smallest terms first),
+// and every column is scaled by a power of two before the split so that its largest entry
+// sits just below 2^14: the FP16 parts stay in range and the scaling is exact.
+//
+// Lane layout of the 16x16x32 f16 product on gfx950 (lane l, i = 0..7 / 0..3):
+//     A: A[l%16][8(l/16)+i]   B: B[8(l/16)+i][l%16]   C: C[4(l/16)+i][l%16]
+// so lane (c = l%16, g = l/16) of a block owns column j = 16*block + c with inputs
+// r = 4g..4g+3 and, after the product, outputs q = 4g..4g+3 of the same column.
+#pragma once
+#include "tile_kernel.h"
+
+namespace gfftx {
+
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+// A operands in device memory: [mt][part][lane][8] with mt = 0 (Re rows) / 1 (Im rows) and
+// part = 0 (hi) / 1 (lo).
+constexpr int MFMA_A_HALFS = 2 * 2 * 64 * 8;
+
+template <int L_, int NB_> struct MfmaCfg {
+    static constexpr int L = L_, NB = NB_, WG = 256;
+    static constexpr int COLS = L / 16;                 // columns per sequence and stage
+    static constexpr int BLOCKS = NB * COLS / 16;       // 16-column blocks per stage (all sequences)
+    static constexpr int BPW = BLOCKS / 4;              // blocks per wave
+    static constexpr int NS = (L == 256) ? 2 : (L == 4096) ? 3 : (L == 65536) ? 4 : 0;
+    static_assert(NS > 0, "length must be 16^S");
+    static_assert(BLOCKS % 4 == 0, "four waves share the blocks evenly");
+    static constexpr int LP = L + L / 16;
+    static constexpr int TS = (LP % 2 == 0) ? LP + 1 : LP;
+    static constexpr int LDS_ELEMS = NB * TS;
+    __device__ static constexpr int lidx(int t, int e) { return t * TS + e + (e >> 4); }
+    static constexpr int ns(int s)
+    {
+        int n = 1;
+        for (int i = 0; i < s; ++i) n *= 16;
+        return n;
+    }
+    static constexpr int twoff(int s)
+    {
+        int o = 0;
+        for (int i = 1; i < s; ++i) o += ns(i) * 16;
+        return o;
+    }
+};
+
+struct MfmaArgs {
+    const float2* in;
+    float2* out;
+    long long ntrans;
+    const float2* tw;       // stage twiddles, rows of 16 entries (same layout as the tile kernel)
+    const _Float16* amat;   // MFMA_A_HALFS halves
+    int conj_in;
+    int conj_out;
+    float scale;
+};
+
+__device__ __forceinline__ float col_max4(float m)
+{
+    // lanes c, c+16, c+32, c+48 hold the same column
+    m = fmaxf(m, __shfl_xor(m, 16, 64));
+    m = fmaxf(m, __shfl_xor(m, 32, 64));
+    return m;
+}
+
+template <class C, int S>
+__device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const half8_t (&A)[2][2], long long t0)
+{
+    constexpr int NSS = C::ns(S);
+    constexpr bool FIRST = (S == 0), LAST = (S == C::NS - 1);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = lane & 15, g = lane >> 4;
+    float2 y[C::BPW][4];
+
+#pragma unroll
+    for (int b = 0; b < C::BPW; ++b) {
+        const int blk = wave * C::BPW + b;             // block index over all NB sequences
+        const int t = blk / (C::COLS / 16);            // sequence within the workgroup
+        const int j = (blk % (C::COLS / 16)) * 16 + c; // column
+        const long long tg = t0 + t;
+        const bool valid = tg < a.ntrans;
+        float2 v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int r = 4 * g + i;
+            const int e = j + r * C::COLS;
+            if constexpr (FIRST) {
+                float2 u = valid ? a.in[tg * C::L + e] : float2{0.f, 0.f};
+                if (a.conj_in) u.y = -u.y;
+                v[i] = u;
+            } else {
+                v[i] = lds[C::lidx(t, e)];
+            }
+        }
+        if constexpr (!FIRST) {
+            const int k = j % NSS;
+            const float2* row = a.tw + C::twoff(S) + k * 16 + 4 * g;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (g != 0 || i != 0) v[i] = cmul(v[i], row[i]);
+        }
+        // per-column power-of-two scale: largest entry just below 2^14
+        float m = 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) m = fmaxf(m, fmaxf(fabsf(v[i].x), fabsf(v[i].y)));
+        m = col_max4(m);
+        int ex = 0;
+        (void)frexpf(m, &ex);
+        const int sh = 14 - ex;
+        half8_t bh, bl;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float xr = ldexpf(v[i].x, sh), xi = ldexpf(v[i].y, sh);
+            const _Float16 hr = (_Float16)xr, hi = (_Float16)xi;
+            bh[2 * i] = hr;
+            bh[2 * i + 1] = hi;
+            bl[2 * i] = (_Float16)(xr - (float)hr);
+            bl[2 * i + 1] = (_Float16)(xi - (float)hi);
+        }
+        float4_t acc[2];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+            float4_t z = {0.f, 0.f, 0.f, 0.f};
+            z = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[mt][1], bh, z, 0, 0, 0);
+            z = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[mt][0], bl, z, 0, 0, 0);
+            z = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[mt][0], bh, z, 0, 0, 0);
+            acc[mt] = z;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) y[b][i] = float2{ldexpf(acc[0][i], -sh), ldexpf(acc[1][i], -sh)};
+    }
+
+    if constexpr (!FIRST) __syncthreads();
+
+#pragma unroll
+    for (int b = 0; b < C::BPW; ++b) {
+        const int blk = wave * C::BPW + b;
+        const int t = blk / (C::COLS / 16);
+        const int j = (blk % (C::COLS / 16)) * 16 + c;
+        const int k = j % NSS;
+        const int base = (j / NSS) * NSS * 16 + k;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int q = 4 * g + i;
+            if constexpr (LAST) {
+                const long long tg = t0 + t;
+                if (tg < a.ntrans) {
+                    float2 u = y[b][i];
+                    if (a.conj_out) u.y = -u.y;
+                    u.x *= a.scale;
+                    u.y *= a.scale;
+                    a.out[tg * C::L + base + q * NSS] = u;
+                }
+            } else {
+                lds[C::lidx(t, base + q * NSS)] = y[b][i];
+            }
+        }
+    }
+    if constexpr (!LAST) __syncthreads();
+}
+
+template <class C, int S>
+__device__ __forceinline__ void mfma_loop(const MfmaArgs& a, float2* lds, const half8_t (&A)[2][2], long long t0)
+{
+    if constexpr (S < C::NS) {
+        mfma_stage<C, S>(a, lds, A, t0);
+        mfma_loop<C, S + 1>(a, lds, A, t0);
+    }
+}
+
+template <class C> __global__ __launch_bounds__(256) void mfma_fft_kernel(MfmaArgs a)
+{
+    __shared__ float2 lds[C::LDS_ELEMS];
+    const int lane = threadIdx.x & 63;
+    half8_t A[2][2];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) A[mt][p][i] = a.amat[((mt * 2 + p) * 64 + lane) * 8 + i];
+    mfma_loop<C, 0>(a, lds, A, (long long)blockIdx.x * C::NB);
+}
+
+}  // namespace gfftx

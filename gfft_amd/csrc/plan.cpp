@@ -86,6 +86,8 @@ struct DevBuf {
 
 struct Pass {
     const TileEntry* k = nullptr;
+    const MfmaEntry* mfma = nullptr;  // split-precision matrix-core pass (then k is unused)
+    void* amat = nullptr;
     long long ntrans = 0, its = 0, ies = 0, ns_pass = 1, bdist = 0;
     unsigned batch = 1;
     void* stage_tw = nullptr;
@@ -328,6 +330,30 @@ int upload_root_tables(std::vector<void*>& owned, long long M, double2** lo_out,
     return GFFTX_SUCCESS;
 }
 
+// A operands of the MFMA radix-16 stage (mfma_kernel.h): [mt][hi/lo][lane][8] halves.
+int upload_mfma_matrix(std::vector<void*>& owned, void** out)
+{
+    std::vector<_Float16> h(MFMA_A_HALFS);
+    for (int mt = 0; mt < 2; ++mt)
+        for (int lane = 0; lane < 64; ++lane)
+            for (int i = 0; i < 8; ++i) {
+                const int q = lane % 16, k = 8 * (lane / 16) + i, r = k / 2, im = k % 2;
+                long double c, sn;
+                root_ld((long long)q * r, 16, &c, &sn);  // F[q][r] = W16^{qr} = c + i sn
+                long double val;
+                if (mt == 0) val = im ? -sn : c;   // Re Y row: Re F, -Im F
+                else val = im ? c : sn;            // Im Y row: Im F,  Re F
+                const _Float16 hi = (_Float16)(double)val;
+                const _Float16 lo = (_Float16)(double)(val - (long double)(double)hi);
+                h[((mt * 2 + 0) * 64 + lane) * 8 + i] = hi;
+                h[((mt * 2 + 1) * 64 + lane) * 8 + i] = lo;
+            }
+    int rc = dev_alloc_list(owned, out, h.size() * sizeof(_Float16));
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(*out, h.data(), h.size() * sizeof(_Float16), hipMemcpyHostToDevice));
+    return GFFTX_SUCCESS;
+}
+
 int upload_pass_table(gfftx_plan_s* p, Pass& ps, long long M)
 {
     ps.ptw_m = M;
@@ -340,6 +366,26 @@ int build_c2c_passes(gfftx_plan_s* p)
     char buf[256];
     p->desc = "{\"n\":" + std::to_string(n) + ",\"batch\":" + std::to_string(p->batch) +
               ",\"precision\":\"" + (p->prec == PREC_F64 ? "f64" : "f32") + "\",\"passes\":[";
+    if (p->flags & GFFTX_FLAG_SPLIT_MFMA) {
+        const MfmaEntry* me = nullptr;
+        for (int i = 0; i < kNumMfma; ++i)
+            if ((unsigned long long)kMfma[i].L == n) me = &kMfma[i];
+        if (p->prec != PREC_F32 || p->real || !me)
+            return fail(GFFTX_ERR_UNSUPPORTED, "split-precision MFMA path: FP32 C2C of length 256 or 4096 only (n=%llu)", n);
+        Pass ps;
+        ps.mfma = me;
+        ps.ntrans = (long long)p->batch;
+        const int rad[4] = {16, 16, 16, 16};
+        int rc = upload_stage_table_r(p->allocs, PREC_F32, me->L == 256 ? 2 : 3, rad, &ps.stage_tw);
+        if (rc) return rc;
+        rc = upload_mfma_matrix(p->allocs, &ps.amat);
+        if (rc) return rc;
+        p->passes.push_back(ps);
+        snprintf(buf, sizeof(buf), "{\"kernel\":\"%s\",\"L\":%d,\"ntrans\":%lld}", me->name, me->L, ps.ntrans);
+        p->desc += buf;
+        p->desc += "]}";
+        return GFFTX_SUCCESS;
+    }
     const TileEntry* single = n <= (1ull << 30) ? find_tile(p->prec, (int)n, false, false) : nullptr;
     if (single) {
         const TileEntry* k = single;
@@ -493,6 +539,25 @@ int run_passes(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
     const void* src = in;
     for (size_t i = 0; i < m; ++i) {
         const Pass& ps = p->passes[i];
+        if (ps.mfma) {
+            MfmaArgs ma;
+            ma.in = (const float2*)src;
+            ma.out = (float2*)dst[i];
+            ma.ntrans = ps.ntrans;
+            ma.tw = (const float2*)ps.stage_tw;
+            ma.amat = (const _Float16*)ps.amat;
+            ma.conj_in = p->inverse ? 1 : 0;
+            ma.conj_out = p->inverse ? 1 : 0;
+            ma.scale = p->inverse ? (float)(1.0 / (double)p->n) : 1.0f;
+            dim3 grid((unsigned)((ps.ntrans + ps.mfma->nb - 1) / ps.mfma->nb));
+            std::pair<hipEvent_t, hipEvent_t> ev;
+            const bool prof = prof_begin(p, i, ps.mfma->name, s, &ev);
+            hipError_t e = ps.mfma->launch(ma, grid, s);
+            if (prof) prof_end(p, i, s, ev);
+            if (e != hipSuccess) return fail(GFFTX_ERR_LAUNCH, "%s: %s", ps.mfma->name, hipGetErrorString(e));
+            src = dst[i];
+            continue;
+        }
         TileArgs a;
         memset(&a, 0, sizeof(a));
         a.in = src;

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "mfma_kernel.h"
 #include "tile_kernel.h"
 
 namespace gfftx {
@@ -26,6 +27,17 @@ struct TileEntry {
 
 extern TileEntry kTiles[];
 extern const int kNumTiles;
+
+// Split-precision MFMA kernels (FP32, lengths 16^S).
+typedef hipError_t (*MfmaLaunch)(const MfmaArgs&, dim3, hipStream_t);
+struct MfmaEntry {
+    int L;
+    int nb;
+    MfmaLaunch launch;
+    const char* name;
+};
+extern MfmaEntry kMfma[];
+extern const int kNumMfma;
 
 // Real-transform split pass (Separate, src/gfftswap.h:239-336) as one kernel.
 struct SplitArgs {
