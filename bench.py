@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--log2n", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-others", action="store_true")
     return ap.parse_args()
 
 
@@ -55,15 +56,17 @@ def round_trip_run(torch, api, n, dtype, steps, warmup, nbuf, seed=24):
     pf = api.Plan(n, vt, api.DFT, api.IN_PLACE, flags=flags)
     pi = api.Plan(n, vt, api.IDFT, api.IN_PLACE, flags=flags)
     stream = torch.cuda.current_stream()
+    # one round trip: the identity up to rounding
+    pf.fft(bufs[0])
+    pi.fft(bufs[0])
+    torch.cuda.synchronize()
+    rt_err = float(np.linalg.norm(bufs[0].cpu().numpy() - x) / np.linalg.norm(x))
+    bufs[0].copy_(torch.from_numpy(x))
     for i in range(warmup):
         b = bufs[i % nbuf]
         pf.fft(b)
         pi.fft(b)
     torch.cuda.synchronize()
-    # round-trip error after warmup round trips (each is the identity up to rounding)
-    rt_err = float(np.linalg.norm(bufs[0].cpu().numpy() - x) / np.linalg.norm(x))
-    for b in bufs:
-        b.copy_(torch.from_numpy(x))
     pf.profile()
     pi.profile()
     torch.cuda.synchronize()
@@ -85,6 +88,94 @@ def round_trip_run(torch, api, n, dtype, steps, warmup, nbuf, seed=24):
     del bufs
     torch.cuda.empty_cache()
     return {"ms": ms, "wall_s": wall, "profile": prof, "rt_err": rt_err, "plan": desc}
+
+
+def _kernel_time_us(plan, run, iters=20, warm=3):
+    """Average duration (us) of each launch slot of `plan` over `iters` calls of run()."""
+    import torch
+
+    for _ in range(warm):
+        run()
+    torch.cuda.synchronize()
+    plan.profile()
+    for _ in range(iters):
+        run()
+    torch.cuda.synchronize()
+    return [(name, tot / cnt * 1e3) for name, tot, cnt in plan.profile()]
+
+
+def other_workloads(torch, api):
+    """The remaining transform workloads, measured in the same run (one line each)."""
+    from gfft_amd import inputs
+
+    res = {}
+    # C1: N=2^10 FP64 in place forward, 2^16 copies per call
+    n, batch = 1024, 65536
+    x = torch.from_numpy(inputs.complex_uniform(n * batch, 17)).cuda()
+    with api.Plan(n, api.DOUBLE, api.DFT, api.IN_PLACE, batch=batch, flags=api.FLAG_PROFILE) as p:
+        ks = _kernel_time_us(p, lambda: p.fft(x))
+    us = sum(t for _, t in ks)
+    res["c1_fp64_1024_x65536"] = {"us": round(us, 1), "GFLOPs": round(5 * n * 10 * batch / us / 1e3, 1),
+                                  "GBps_alg": round(2 * n * batch * 16 / us / 1e3, 1)}
+    del x
+    # C3: R2C N = 3*2^20 real FP32 -> N/2+1 bins
+    nr = 3 * 2 ** 20
+    xr, _, _ = inputs.real_sinusoids(nr, 3)
+    z = torch.from_numpy(xr.astype(np.float32)).cuda()
+    o = torch.zeros(nr // 2 + 1, dtype=torch.complex64, device="cuda")
+    with api.Plan(nr // 2, api.FLOAT, api.RDFT, api.OUT_OF_PLACE, flags=api.FLAG_R2C_BINS | api.FLAG_PROFILE) as p:
+        ks = _kernel_time_us(p, lambda: p.fft(z, o))
+    us = sum(t for _, t in ks)
+    res["c3_r2c_3x2p20_f32"] = {"us": round(us, 1),
+                                "GFLOPs_2.5NlogN": round(2.5 * nr * np.log2(nr) / us / 1e3, 1),
+                                "GBps_alg": round((4 * nr + 8 * (nr // 2 + 1)) / us / 1e3, 1),
+                                "kernels": [(k, round(t, 1)) for k, t in ks]}
+    del z, o
+    # C4: ragged batch of 4096 FP32 transforms, one persistent launch
+    lengths = [int(v) for v in inputs.ragged_lengths(4096, 4096)]
+    tot = sum(lengths)
+    xg = torch.from_numpy(inputs.complex_normal(tot, 4096).astype(np.complex64)).cuda()
+    rp = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT, flags=api.FLAG_PROFILE | api.FLAG_GRAPH)
+    for _ in range(3):
+        rp.fft(xg)
+    torch.cuda.synchronize()
+    rp.profile()
+    for _ in range(20):
+        rp.fft(xg)
+    ms, cnt = rp.profile()
+    us = ms / cnt * 1e3
+    flops = sum(5 * L * np.log2(L) for L in lengths)
+    res["c4_ragged_4096_f32"] = {"us": round(us, 1), "GFLOPs": round(flops / us / 1e3, 1),
+                                 "GBps_alg": round(2 * tot * 8 / us / 1e3, 1), "points": tot}
+    rp.close()
+    del xg
+    # C5: 16384 x 4096 FP32 plain vs split-precision MFMA; accuracy against the FP64 engine
+    n, batch = 4096, 16384
+    xs = inputs.sinusoid_batch(batch, n, 5)
+    x32 = torch.from_numpy(xs.astype(np.complex64)).cuda()
+    out = torch.empty_like(x32)
+    c5 = {}
+    for label, flags in (("plain", 0), ("mfma", api.FLAG_SPLIT_MFMA)):
+        with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE, batch=batch,
+                      flags=flags | api.FLAG_PROFILE) as p:
+            ks = _kernel_time_us(p, lambda: p.fft(x32, out))
+            rows = out.cpu().numpy()[::64]
+        us = sum(t for _, t in ks)
+        c5[label] = {"kernel": ks[0][0], "us": round(us, 1),
+                     "GFLOPs": round(5 * n * 12 * batch / us / 1e3, 1),
+                     "GBps_alg": round(2 * n * batch * 8 / us / 1e3, 1), "rows": rows}
+    del x32, out
+    x64 = torch.from_numpy(xs[::64].astype(np.complex64).astype(np.complex128)).cuda()
+    with api.Plan(n, api.COMPLEX_DOUBLE, api.DFT, api.IN_PLACE, batch=x64.shape[0]) as p:
+        p.fft(x64)
+    ref = x64.cpu().numpy()
+    for label in ("plain", "mfma"):
+        r = c5[label].pop("rows")
+        errs = np.linalg.norm(r - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        c5[label]["relL2_max_vs_fp64"] = float(errs.max())
+    c5["mfma_over_plain"] = round(c5["plain"]["us"] / c5["mfma"]["us"], 3)
+    res["c5_16384x4096_f32"] = c5
+    return res
 
 
 def cpu_baseline(n_log2: int):
@@ -171,6 +262,8 @@ def main():
         "roofline": roofline,
         "kernels": per_kernel,
     }
+    if not args.no_others:
+        out["workloads"] = other_workloads(torch, api)
     if not args.no_fp64:
         nbuf64 = max(2, math.ceil(512 * 2 ** 20 / (n * 16)))
         r64 = round_trip_run(torch, api, n, np.complex128, max(10, args.steps // 2),
