@@ -206,8 +206,8 @@ def cpu_baseline(n_log2: int):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the transform shards as described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: every "
+              "workload here runs on a single MI355X", file=sys.stderr)
         sys.exit(2)
     import torch
 

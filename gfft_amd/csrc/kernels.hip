@@ -58,6 +58,35 @@ template <class C> static hipError_t launch_tile(const TileArgs& a, dim3 grid, h
     GFFTX_TILE(PREC, TYPE, 5120, 1, 320, false, false, 16, 16, 4, 5),       \
     GFFTX_TILE(PREC, TYPE, 6144, 1, 384, false, false, 16, 16, 8, 3)
 
+// Pass tiles carrying the factors 3 and 5 (T->E and T->T), so that every 2^a 3^b 5^c
+// length splits into compiled passes.
+#define GFFTX_PASS_MIXED(PREC, TYPE)                                          \
+    GFFTX_TILE(PREC, TYPE, 48, 85, 255, true, false, 16, 3),                 \
+    GFFTX_TILE(PREC, TYPE, 48, 85, 255, true, true, 16, 3),                  \
+    GFFTX_TILE(PREC, TYPE, 80, 51, 255, true, false, 16, 5),                 \
+    GFFTX_TILE(PREC, TYPE, 80, 51, 255, true, true, 16, 5),                  \
+    GFFTX_TILE(PREC, TYPE, 192, 21, 252, true, false, 16, 4, 3),             \
+    GFFTX_TILE(PREC, TYPE, 192, 21, 252, true, true, 16, 4, 3),              \
+    GFFTX_TILE(PREC, TYPE, 320, 12, 240, true, false, 16, 4, 5),             \
+    GFFTX_TILE(PREC, TYPE, 320, 12, 240, true, true, 16, 4, 5),              \
+    GFFTX_TILE(PREC, TYPE, 768, 5, 240, true, false, 16, 16, 3),             \
+    GFFTX_TILE(PREC, TYPE, 768, 5, 240, true, true, 16, 16, 3),              \
+    GFFTX_TILE(PREC, TYPE, 1280, 3, 240, true, false, 16, 16, 5),            \
+    GFFTX_TILE(PREC, TYPE, 1280, 3, 240, true, true, 16, 16, 5)
+
+// Small pass tiles (one or two stages) that absorb leftover factors in one extra pass.
+#define GFFTX_PASS_SMALL_ONE(PREC, TYPE, ST)                                  \
+    GFFTX_TILE(PREC, TYPE, 2, 256, 256, true, ST, 2),                        \
+    GFFTX_TILE(PREC, TYPE, 3, 256, 256, true, ST, 3),                        \
+    GFFTX_TILE(PREC, TYPE, 4, 256, 256, true, ST, 4),                        \
+    GFFTX_TILE(PREC, TYPE, 5, 256, 256, true, ST, 5),                        \
+    GFFTX_TILE(PREC, TYPE, 8, 256, 256, true, ST, 8),                        \
+    GFFTX_TILE(PREC, TYPE, 9, 85, 255, true, ST, 3, 3),                      \
+    GFFTX_TILE(PREC, TYPE, 15, 85, 255, true, ST, 5, 3),                     \
+    GFFTX_TILE(PREC, TYPE, 25, 51, 255, true, ST, 5, 5),                     \
+    GFFTX_TILE(PREC, TYPE, 45, 28, 252, true, ST, 5, 3, 3)
+#define GFFTX_PASS_SMALL(PREC, TYPE) GFFTX_PASS_SMALL_ONE(PREC, TYPE, false), GFFTX_PASS_SMALL_ONE(PREC, TYPE, true)
+
 // Pass tiles for multi-pass transforms: first pass (T->E) and later passes (T->T).
 #define GFFTX_PASS_SET(PREC, TYPE, NB4096, WG4096, NB1024, WG1024, NB1536, WG1536)         \
     GFFTX_TILE(PREC, TYPE, 256, 16, 256, true, false, 16, 16),                             \
@@ -86,6 +115,10 @@ TileEntry kTiles[] = {
     GFFTX_TILE(PREC_F32, float, 256, 32, 512, true, true, 16, 16),
     GFFTX_TILE(PREC_F32, float, 16, 64, 64, true, true, 16),
     GFFTX_TILE(PREC_F32, float, 64, 64, 256, true, true, 16, 4),
+    GFFTX_TILE(PREC_F32, float, 16, 64, 64, true, false, 16),
+    GFFTX_TILE(PREC_F32, float, 64, 64, 256, true, false, 16, 4),
+    GFFTX_PASS_MIXED(PREC_F32, float),
+    GFFTX_PASS_SMALL(PREC_F32, float),
     GFFTX_BATCH_POW2(PREC_F64, double),
     GFFTX_BATCH_MIXED(PREC_F64, double),
     GFFTX_PASS_SET(PREC_F64, double, 2, 512, 4, 256, 4, 384),
@@ -93,6 +126,10 @@ TileEntry kTiles[] = {
     GFFTX_TILE(PREC_F64, double, 4096, 1, 256, true, true, 16, 16, 16),
     GFFTX_TILE(PREC_F64, double, 16, 64, 64, true, true, 16),
     GFFTX_TILE(PREC_F64, double, 64, 64, 256, true, true, 16, 4),
+    GFFTX_TILE(PREC_F64, double, 16, 64, 64, true, false, 16),
+    GFFTX_TILE(PREC_F64, double, 64, 64, 256, true, false, 16, 4),
+    GFFTX_PASS_MIXED(PREC_F64, double),
+    GFFTX_PASS_SMALL(PREC_F64, double),
 };
 const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 

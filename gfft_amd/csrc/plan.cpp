@@ -132,6 +132,7 @@ bool only_235(unsigned long long n)
 double pass_bandwidth(int prec, int L)
 {
     const bool f32 = prec == PREC_F32;
+    if (L < 16) return 5.0;  // one extra pass over the data for a leftover factor
     if (L <= 16) return f32 ? 5.5 : 5.6;
     if (L <= 64) return f32 ? 4.6 : 4.7;
     if (L <= 256) return f32 ? 4.2 : 4.6;
@@ -176,12 +177,20 @@ bool choose_passes(int prec, unsigned long long n, std::vector<int>& out)
     double best_cost = 1e300;
     std::function<void(unsigned long long, int)> rec = [&](unsigned long long rem, int maxi) {
         if (rem == 1) {
-            if (cur.size() < 2 || !find_tile(prec, cur[0], true, false)) return;
+            if (cur.size() < 2) return;
             double cost = 0;
             for (int v : cur) cost += 1.0 / pass_bandwidth(prec, v);
             if (cost < best_cost - 1e-12) {
+                // the first pass needs a T->E tile: rotate one in front when possible
+                std::vector<int> order = cur;
+                for (size_t i = 0; i < order.size(); ++i)
+                    if (find_tile(prec, order[i], true, false)) {
+                        std::swap(order[0], order[i]);
+                        break;
+                    }
+                if (!find_tile(prec, order[0], true, false)) return;
                 best_cost = cost;
-                best = cur;
+                best = order;
             }
             return;
         }
@@ -225,6 +234,13 @@ struct gfftx_plan_s {
         std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
     };
     std::vector<Slot> slots;
+    bool capturing = false;
+    // GFFTX_FLAG_GRAPH: launches captured once per (in, out) pair and replayed
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    const void* graph_in = nullptr;
+    void* graph_out = nullptr;
 };
 
 namespace {
@@ -499,7 +515,7 @@ bool is_device_ptr(const void* ptr)
 // Event pair for launch slot `slot` (GFFTX_FLAG_PROFILE); returns false when not profiling.
 bool prof_begin(gfftx_plan_s* p, size_t slot, const char* name, hipStream_t s, std::pair<hipEvent_t, hipEvent_t>* ev)
 {
-    if (!(p->flags & GFFTX_FLAG_PROFILE)) return false;
+    if (!(p->flags & GFFTX_FLAG_PROFILE) || p->capturing) return false;
     if (p->slots.size() <= slot) p->slots.resize(slot + 1);
     gfftx_plan_s::Slot& sl = p->slots[slot];
     sl.name = name;
@@ -620,10 +636,6 @@ int exec_device(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
     }
     void* work = out;
     if (in != out) {
-        if (!p->real_work) {
-            int rc = dev_alloc(p, &p->real_work, bytes);
-            if (rc) return rc;
-        }
         work = p->real_work;
         HIP_TRY(hipMemcpyAsync(work, in, bytes, hipMemcpyDeviceToDevice, s));
     }
@@ -632,11 +644,42 @@ int exec_device(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
     return run_passes(p, work, out, s);
 }
 
+// Replays the plan's launches as one captured hipGraph (captured again when the buffers
+// change).  Capture happens on a plan-owned stream; the graph is launched on the caller's.
+int exec_graph(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
+{
+    if (!p->graph_exec || p->graph_in != in || p->graph_out != out) {
+        if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+        if (p->graph) (void)hipGraphDestroy(p->graph);
+        p->graph_exec = nullptr;
+        p->graph = nullptr;
+        if (!p->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+        HIP_TRY(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+        p->capturing = true;
+        const int rc = exec_device(p, in, out, p->cap_stream);
+        p->capturing = false;
+        hipGraph_t gr = nullptr;
+        const hipError_t ce = hipStreamEndCapture(p->cap_stream, &gr);
+        if (rc) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return rc;
+        }
+        if (ce != hipSuccess) return fail(GFFTX_ERR_DEVICE, "end capture: %s", hipGetErrorString(ce));
+        p->graph = gr;
+        HIP_TRY(hipGraphInstantiate(&p->graph_exec, gr, nullptr, nullptr, 0));
+        p->graph_in = in;
+        p->graph_out = out;
+    }
+    HIP_TRY(hipGraphLaunch(p->graph_exec, s));
+    return GFFTX_SUCCESS;
+}
+
 int exec_any(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
 {
     DeviceGuard g(p->device);
     if (!g.ok) return fail(GFFTX_ERR_DEVICE, "cannot select device %d", p->device);
     const bool din = is_device_ptr(in), dout = is_device_ptr(out);
+    if (din && dout && (p->flags & GFFTX_FLAG_GRAPH)) return exec_graph(p, in, out, s);
     // complex elements the caller's arrays hold: the real side of a real transform has n
     // per item, the complex side dist (n, or n+1 with the bins layout)
     const size_t full = (size_t)p->dist * p->batch, real_side = (size_t)p->dist * (p->batch - 1) + p->n;
@@ -721,6 +764,8 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
     p->dist = n + ((real && (flags & GFFTX_FLAG_R2C_BINS)) ? 1 : 0);
     int rc = build_c2c_passes(p);
     if (rc == GFFTX_SUCCESS && real) rc = upload_split_table(p);
+    if (rc == GFFTX_SUCCESS && real && p->inverse)  // split pass of an out-of-place IRDFT works on a copy
+        rc = dev_alloc(p, &p->real_work, (size_t)p->dist * batch * elem_bytes(p->prec));
     if (rc == GFFTX_SUCCESS && p->passes.size() > 1) {
         const size_t bytes = (size_t)p->dist * batch * elem_bytes(p->prec);
         rc = dev_alloc(p, &p->scratch[0], bytes);
@@ -1014,6 +1059,9 @@ int gfftx_plan_destroy(gfftx_plan plan)
 {
     if (!plan) return GFFTX_SUCCESS;
     DeviceGuard g(plan->device);
+    if (plan->graph_exec) (void)hipGraphExecDestroy(plan->graph_exec);
+    if (plan->graph) (void)hipGraphDestroy(plan->graph);
+    if (plan->cap_stream) (void)hipStreamDestroy(plan->cap_stream);
     for (auto& sl : plan->slots) {
         for (auto& ev : sl.pending) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
         for (auto& ev : sl.pool) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }

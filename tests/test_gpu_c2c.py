@@ -89,7 +89,8 @@ def test_large_single_pass_fp32(torch_cuda, n):
 
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
-@pytest.mark.parametrize("n", [3 * 2 ** 19, 1536 * 256])
+@pytest.mark.parametrize("n", [3 * 2 ** 19, 1536 * 256, 5 * 2 ** 16, 15 * 2 ** 12, 9 * 2 ** 14, 25 * 2 ** 12,
+                               45 * 2 ** 10, 3 * 5 * 2 ** 18])
 def test_mixed_radix_multi_pass(torch_cuda, dtype, n):
     x = inputs.complex_normal(n, 3).astype(dtype)
     with api.Plan(n, VT[dtype], api.DFT, api.OUT_OF_PLACE) as p:
@@ -174,6 +175,28 @@ def test_host_pointers_are_synchronous(torch_cuda):
         assert rel_l2(out, ref) <= 1e-13
 
 
+@pytest.mark.parametrize("n,vt,tt", [(2 ** 20, api.COMPLEX_FLOAT, api.DFT), (4096, api.COMPLEX_DOUBLE, api.IDFT),
+                                     (3 * 2 ** 12, api.FLOAT, api.RDFT), (2 ** 16, api.DOUBLE, api.IRDFT)])
+def test_graph_replays_match_eager(torch_cuda, n, vt, tt):
+    """GFFTX_FLAG_GRAPH: several replays, with eager work and synchronisation between them,
+    each equal to the eager result."""
+    dt = np.complex64 if vt in (api.FLOAT, api.COMPLEX_FLOAT) else np.complex128
+    x = inputs.complex_normal(n, 31).astype(dt)
+    with api.Plan(n, vt, tt, api.OUT_OF_PLACE) as pe, api.Plan(n, vt, tt, api.OUT_OF_PLACE, flags=api.FLAG_GRAPH) as pg:
+        s = dev(torch_cuda, x)
+        o = torch_cuda.empty_like(s)
+        pe.fft(s, o)
+        ref = host(o)
+        og = torch_cuda.empty_like(s)
+        for rep in range(4):
+            og.zero_()
+            pg.fft(s, og)
+            torch_cuda.cuda.synchronize()
+            _ = float(s.abs().sum())
+            assert np.array_equal(host(og), ref), f"replay {rep}"
+            assert np.array_equal(host(s), x)
+
+
 def test_out_of_place_alias_rejected(torch_cuda):
     with api.Plan(256, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE) as p:
         d = torch_cuda.zeros(256, dtype=torch_cuda.complex64, device="cuda")
@@ -202,6 +225,17 @@ def test_multi_pass(torch_cuda, dtype, n):
         o = torch_cuda.empty_like(s)
         p.fft(s, o)
         assert rel_l2(host(o), x) <= 2 * TOL[dtype]
+
+
+def test_very_long_fp32(torch_cuda):
+    """2^26 FP32 forward against the oracle (four passes or fewer)."""
+    n = 2 ** 26
+    x = inputs.complex_normal(n, 26).astype(np.complex64)
+    with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.IN_PLACE) as p:
+        d = dev(torch_cuda, x)
+        p.fft(d)
+        y = host(d)
+    assert rel_l2(y, oracle_fwd(x)) <= 2e-7
 
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
