@@ -17,6 +17,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <map>
+#include <sys/stat.h>
+#include <unistd.h>
+#include <cctype>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -140,6 +144,51 @@ double pass_bandwidth(int prec, int L)
     return f32 ? 2.6 : 2.5;
 }
 
+// Pass splits of n into compiled T-mode tile lengths (2..4 passes), best `k` by the cost
+// model: each pass moves 2 n b_c bytes at the bandwidth its tile reaches
+// (pass_bandwidth()).  The first length of every split has a T->E tile.
+std::vector<std::vector<int>> candidate_passes(int prec, unsigned long long n, size_t k)
+{
+    std::vector<int> lens;
+    for (int i = 0; i < kNumTiles; ++i)
+        if (kTiles[i].prec == prec && kTiles[i].loadT && kTiles[i].storeT) lens.push_back(kTiles[i].L);
+    std::sort(lens.begin(), lens.end());
+    lens.erase(std::unique(lens.begin(), lens.end()), lens.end());
+    std::vector<std::pair<double, std::vector<int>>> found;
+    std::vector<int> cur;
+    std::function<void(unsigned long long, int)> rec = [&](unsigned long long rem, int maxi) {
+        if (rem == 1) {
+            if (cur.size() < 2) return;
+            std::vector<int> order = cur;
+            for (size_t i = 0; i < order.size(); ++i)
+                if (find_tile(prec, order[i], true, false)) {
+                    std::swap(order[0], order[i]);
+                    break;
+                }
+            if (!find_tile(prec, order[0], true, false)) return;
+            double cost = 0;
+            for (int v : order) cost += 1.0 / pass_bandwidth(prec, v);
+            found.push_back({cost, order});
+            return;
+        }
+        if (cur.size() == 4) return;
+        for (int i = maxi; i >= 0; --i) {
+            if (rem % (unsigned long long)lens[i]) continue;
+            cur.push_back(lens[i]);
+            rec(rem / lens[i], i);
+            cur.pop_back();
+        }
+    };
+    rec(n, (int)lens.size() - 1);
+    std::stable_sort(found.begin(), found.end(),
+                     [](const std::pair<double, std::vector<int>>& x, const std::pair<double, std::vector<int>>& y) {
+                         return x.first < y.first;
+                     });
+    std::vector<std::vector<int>> out;
+    for (size_t i = 0; i < found.size() && out.size() < k; ++i) out.push_back(found[i].second);
+    return out;
+}
+
 // Split n into pass lengths available as T-mode tiles (largest first).  An override list
 // (GFFTX_PASSES="a,b,c") is honoured when it multiplies to n.
 bool choose_passes(int prec, unsigned long long n, std::vector<int>& out)
@@ -165,46 +214,9 @@ bool choose_passes(int prec, unsigned long long n, std::vector<int>& out)
             return true;
         }
     }
-    std::vector<int> lens;
-    for (int i = 0; i < kNumTiles; ++i)
-        if (kTiles[i].prec == prec && kTiles[i].loadT && kTiles[i].storeT) lens.push_back(kTiles[i].L);
-    std::sort(lens.begin(), lens.end());
-    lens.erase(std::unique(lens.begin(), lens.end()), lens.end());
-    // Enumerate non-increasing sequences of pass lengths with product n (2..4 passes) and
-    // keep the one with the lowest modelled time: each pass moves 2 n b_c bytes at the
-    // bandwidth its tile reaches (pass_bandwidth()).
-    std::vector<int> best, cur;
-    double best_cost = 1e300;
-    std::function<void(unsigned long long, int)> rec = [&](unsigned long long rem, int maxi) {
-        if (rem == 1) {
-            if (cur.size() < 2) return;
-            double cost = 0;
-            for (int v : cur) cost += 1.0 / pass_bandwidth(prec, v);
-            if (cost < best_cost - 1e-12) {
-                // the first pass needs a T->E tile: rotate one in front when possible
-                std::vector<int> order = cur;
-                for (size_t i = 0; i < order.size(); ++i)
-                    if (find_tile(prec, order[i], true, false)) {
-                        std::swap(order[0], order[i]);
-                        break;
-                    }
-                if (!find_tile(prec, order[0], true, false)) return;
-                best_cost = cost;
-                best = order;
-            }
-            return;
-        }
-        if (cur.size() == 4) return;
-        for (int i = maxi; i >= 0; --i) {
-            if (rem % (unsigned long long)lens[i]) continue;
-            cur.push_back(lens[i]);
-            rec(rem / lens[i], i);
-            cur.pop_back();
-        }
-    };
-    rec(n, (int)lens.size() - 1);
-    if (best.empty()) return false;
-    out = best;
+    std::vector<std::vector<int>> c = candidate_passes(prec, n, 1);
+    if (c.empty()) return false;
+    out = c[0];
     return true;
 }
 
@@ -220,6 +232,8 @@ struct gfftx_plan_s {
     bool real = false;
     bool inverse = false;
     unsigned long long dist = 0;   // complex elements between batch items
+    std::vector<int> forced_lens;  // pass split chosen by the autotuner (empty: cost model)
+    std::string tune_note;         // autotuner record for plan_describe
     std::vector<Pass> passes;
     std::vector<void*> allocs;     // device allocations owned by the plan
     void* scratch[2] = {nullptr, nullptr};
@@ -419,8 +433,8 @@ int build_c2c_passes(gfftx_plan_s* p)
         snprintf(buf, sizeof(buf), "{\"kernel\":\"%s\",\"L\":%d,\"ntrans\":%lld}", k->name, k->L, ps.ntrans);
         p->desc += buf;
     } else {
-        std::vector<int> lens;
-        if (!choose_passes(p->prec, n, lens))
+        std::vector<int> lens = p->forced_lens;
+        if (lens.empty() && !choose_passes(p->prec, n, lens))
             return fail(GFFTX_ERR_UNSUPPORTED, "no multi-pass decomposition for n=%llu", n);
         long long ns = 1;
         for (size_t i = 0; i < lens.size(); ++i) {
@@ -442,14 +456,14 @@ int build_c2c_passes(gfftx_plan_s* p)
                 rc = upload_pass_table(p, ps, ns * L);
                 if (rc) return rc;
             }
-            p->passes.push_back(ps);
+                p->passes.push_back(ps);
             snprintf(buf, sizeof(buf), "%s{\"kernel\":\"%s\",\"L\":%d,\"ns\":%lld,\"ntrans\":%lld,\"xcd\":%d}",
                      i ? "," : "", k->name, L, ns, ps.ntrans, ps.xcd);
             p->desc += buf;
             ns *= L;
         }
     }
-    p->desc += "]}";
+    p->desc += "]" + p->tune_note + "}";
     return GFFTX_SUCCESS;
 }
 
@@ -592,7 +606,8 @@ int run_passes(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
         a.scale = (p->inverse && i == m - 1) ? 1.0 / (double)p->n : 1.0;
         a.xcd = ps.xcd;
         const long long nb = ps.k->nb;
-        dim3 grid((unsigned)((ps.ntrans + nb - 1) / nb), ps.batch);
+        const unsigned ntiles = (unsigned)((ps.ntrans + nb - 1) / nb);
+        dim3 grid(ntiles, ps.batch);
         std::pair<hipEvent_t, hipEvent_t> ev;
         const bool prof = prof_begin(p, i, ps.k->name, s, &ev);
         hipError_t e = ps.k->launch(a, grid, s);
@@ -718,8 +733,8 @@ int exec_any(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
     return GFFTX_SUCCESS;
 }
 
-int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttype, int place, int device,
-                uint32_t flags)
+int create_plan_impl(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttype, int place, int device,
+                     uint32_t flags, const std::vector<int>& forced, const std::string& note)
 {
     if (!out) return fail(GFFTX_ERR_INVALID_ARG, "plan pointer is NULL");
     *out = nullptr;
@@ -762,6 +777,8 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
     p->real = real;
     p->inverse = ttype == GFFTX_IDFT || ttype == GFFTX_IRDFT;
     p->dist = n + ((real && (flags & GFFTX_FLAG_R2C_BINS)) ? 1 : 0);
+    p->forced_lens = forced;
+    p->tune_note = note;
     int rc = build_c2c_passes(p);
     if (rc == GFFTX_SUCCESS && real) rc = upload_split_table(p);
     if (rc == GFFTX_SUCCESS && real && p->inverse)  // split pass of an out-of-place IRDFT works on a copy
@@ -995,6 +1012,193 @@ int ragged_launch(gfftx_ragged_s* r, void* data, hipStream_t s)
     return GFFTX_SUCCESS;
 }
 
+
+// ---------------------------------------------------------------------------------------
+// Autotuner: for a multi-pass plan, time the best few pass splits of the cost model on this
+// device and keep the fastest.  Results are cached per (device, precision, n, batch) in
+// memory and in a versioned JSON file ($GFFTX_TUNE_CACHE, default ~/.cache/gfftx/tune.json).
+std::mutex g_tune_mu;
+std::map<std::string, std::vector<int>> g_tune;
+std::string g_tune_loaded;  // cache file the in-memory entries belong to
+
+std::string tune_path()
+{
+    if (const char* e = getenv("GFFTX_TUNE_CACHE")) return e;
+    const char* h = getenv("HOME");
+    return std::string(h ? h : "/tmp") + "/.cache/gfftx/tune.json";
+}
+
+// One entry per line: "key": [a, b, c]
+void tune_load()
+{
+    const std::string path = tune_path();
+    if (g_tune_loaded == path) return;
+    g_tune_loaded = path;
+    g_tune.clear();
+    FILE* f = fopen(path.c_str(), "r");
+    if (!f) return;
+    char line[512];
+    while (fgets(line, sizeof(line), f)) {
+        char* q1 = strchr(line, '"');
+        if (!q1) continue;
+        char* q2 = strchr(q1 + 1, '"');
+        char* lb = q2 ? strchr(q2, '[') : nullptr;
+        if (!lb) continue;
+        const std::string key(q1 + 1, q2);
+        if (key == "version" || key == "entries") continue;
+        std::vector<int> v;
+        char* p = lb + 1;
+        while (*p && *p != ']') {
+            const long x = strtol(p, &p, 10);
+            if (x > 1) v.push_back((int)x);
+            while (*p == ',' || *p == ' ') ++p;
+            if (!isdigit((unsigned char)*p) && *p != ']') break;
+        }
+        if (v.size() >= 2) g_tune[key] = v;
+    }
+    fclose(f);
+}
+
+void tune_store()
+{
+    const std::string path = tune_path();
+    const size_t slash = path.find_last_of('/');
+    if (slash != std::string::npos) {
+        std::string dir = path.substr(0, slash);
+        for (size_t i = 1; i <= dir.size(); ++i)
+            if (i == dir.size() || dir[i] == '/') (void)mkdir(dir.substr(0, i).c_str(), 0755);
+    }
+    const std::string tmp = path + ".tmp" + std::to_string((long)getpid());
+    FILE* f = fopen(tmp.c_str(), "w");
+    if (!f) return;
+    fprintf(f, "{\"version\": 1, \"entries\": {\n");
+    size_t i = 0;
+    for (const auto& kv : g_tune) {
+        fprintf(f, "\"%s\": [", kv.first.c_str());
+        for (size_t j = 0; j < kv.second.size(); ++j) fprintf(f, "%s%d", j ? ", " : "", kv.second[j]);
+        fprintf(f, "]%s\n", ++i < g_tune.size() ? "," : "");
+    }
+    fprintf(f, "}}\n");
+    fclose(f);
+    (void)rename(tmp.c_str(), path.c_str());
+}
+
+std::string tune_key(int device, int prec, uint64_t n, uint64_t batch)
+{
+    hipDeviceProp_t prop;
+    std::string dev = "dev";
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) dev = std::string(prop.gcnArchName) + ":" + std::to_string(prop.multiProcessorCount) + "cu";
+    else (void)hipGetLastError();
+    return dev + "|" + (prec == PREC_F64 ? "f64" : "f32") + "|n=" + std::to_string(n) + "|b=" + std::to_string(batch);
+}
+
+// Median of `reps` timed executions (ms) of a plan on scratch buffers.
+int time_plan(gfftx_plan_s* p, void* a, void* b, hipStream_t s, int reps, float* ms_out)
+{
+    for (int i = 0; i < 2; ++i) {
+        int rc = exec_device(p, a, b, s);
+        if (rc) return rc;
+    }
+    std::vector<float> t;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    for (int i = 0; i < reps; ++i) {
+        (void)hipEventRecord(e0, s);
+        int rc = exec_device(p, a, b, s);
+        (void)hipEventRecord(e1, s);
+        if (rc) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return rc;
+        }
+        (void)hipEventSynchronize(e1);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        t.push_back(ms);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::sort(t.begin(), t.end());
+    *ms_out = t[t.size() / 2];
+    return GFFTX_SUCCESS;
+}
+
+int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttype, int place, int device,
+                uint32_t flags)
+{
+    int rc = create_plan_impl(out, n, batch, vtype, ttype, place, device, flags, {}, "");
+    if (rc != GFFTX_SUCCESS) return rc;
+    gfftx_plan_s* p = *out;
+    if (p->passes.size() < 2 || (flags & GFFTX_FLAG_NO_AUTOTUNE) || env_int("GFFTX_AUTOTUNE", 1) == 0 ||
+        getenv("GFFTX_PASSES"))
+        return GFFTX_SUCCESS;
+    const std::string key = tune_key(device, p->prec, n, batch);
+    std::vector<int> chosen;
+    std::string note;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        tune_load();
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            chosen = it->second;
+            note = ",\"tuned\":\"cache\"";
+        }
+    }
+    if (chosen.empty()) {
+        std::vector<std::vector<int>> cands = candidate_passes(p->prec, n, (size_t)env_int("GFFTX_TUNE_CANDIDATES", 5));
+        if (cands.size() < 2) return GFFTX_SUCCESS;
+        DeviceGuard g(device);
+        const size_t bytes = (size_t)p->dist * batch * elem_bytes(p->prec);
+        void *a = nullptr, *b = nullptr;
+        if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            if (a) (void)hipFree(a);
+            return GFFTX_SUCCESS;  // no room to tune: keep the cost-model plan
+        }
+        hipStream_t s = nullptr;
+        (void)hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        (void)hipMemsetAsync(a, 0, bytes, s);
+        float best = 1e30f;
+        std::string timings = ",\"tuned\":[";
+        for (size_t i = 0; i < cands.size(); ++i) {
+            gfftx_plan cp = nullptr;
+            if (create_plan_impl(&cp, n, batch, vtype, GFFTX_DFT, GFFTX_OUT_OF_PLACE, device,
+                                 flags & GFFTX_FLAG_R2C_BINS, cands[i], "") != GFFTX_SUCCESS)
+                continue;
+            float ms = 0;
+            const int trc = time_plan(cp, a, b, s, 5, &ms);
+            gfftx_plan_destroy(cp);
+            if (trc) continue;
+            std::string lens;
+            for (size_t j = 0; j < cands[i].size(); ++j) lens += (j ? "x" : "") + std::to_string(cands[i][j]);
+            char buf[96];
+            snprintf(buf, sizeof(buf), "%s{\"split\":\"%s\",\"ms\":%.4f}", i ? "," : "", lens.c_str(), ms);
+            timings += buf;
+            if (ms < best) {
+                best = ms;
+                chosen = cands[i];
+            }
+        }
+        timings += "]";
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+        (void)hipFree(a);
+        (void)hipFree(b);
+        (void)hipGetLastError();
+        if (chosen.empty()) return GFFTX_SUCCESS;
+        note = timings;
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[key] = chosen;
+        tune_store();
+    }
+    gfftx_plan q = nullptr;
+    rc = create_plan_impl(&q, n, batch, vtype, ttype, place, device, flags, chosen, note);
+    if (rc != GFFTX_SUCCESS) return GFFTX_SUCCESS;  // keep the cost-model plan
+    gfftx_plan_destroy(p);
+    *out = q;
+    return GFFTX_SUCCESS;
+}
 }  // namespace
 
 extern "C" {

@@ -119,7 +119,35 @@ __device__ __forceinline__ double2 pass_root(const TileArgs& a, long long m)
     return dmul(lo, hi);
 }
 
-template <class C, int S> __device__ __forceinline__ void tile_stage(Ctx<C>& x)
+// First-stage inputs of one thread (raw loads, conjugated for inverse transforms).
+template <class C> struct FirstIn {
+    typename C::V v[C::nbf(0)][C::R[0]];
+};
+
+template <class C> __device__ __forceinline__ void load_first(const Ctx<C>& x, FirstIn<C>& f)
+{
+    using V = typename C::V;
+    constexpr int R = C::R[0];
+    constexpr int B = C::nbf(0);
+    constexpr int LR = C::L / R;
+    constexpr bool GUARD = (B * C::TPT > LR);
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int j = x.jt + b * C::TPT;
+        if (!x.active || (GUARD && j >= LR)) continue;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            V u = V{0, 0};
+            if (x.valid) {
+                u = x.in[x.base_in + (long long)(j + r * LR) * x.a.ies];
+                if (x.a.conj_in) u.y = -u.y;
+            }
+            f.v[b][r] = u;
+        }
+    }
+}
+
+template <class C, int S> __device__ __forceinline__ void tile_stage(Ctx<C>& x, FirstIn<C>& f)
 {
     using V = typename C::V;
     constexpr int R = C::R[S];
@@ -137,14 +165,9 @@ template <class C, int S> __device__ __forceinline__ void tile_stage(Ctx<C>& x)
         if (!x.active || (GUARD && j >= LR)) continue;
         if constexpr (FIRST) {
             const TileArgs& a = x.a;
-            if (x.valid) {
 #pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const long long e = j + r * LR;
-                    V u = x.in[x.base_in + e * a.ies];
-                    if (a.conj_in) u.y = -u.y;
-                    v[b][r] = u;
-                }
+            for (int r = 0; r < R; ++r) v[b][r] = f.v[b][r];
+            if (x.valid) {
                 if (a.ptw_lo != nullptr) {
                     // W_M^{kp*e}, e = j + r*LR: base W^{kp*j} times step^r, step = W^{kp*LR}
                     const long long kp = x.tg % a.ns_pass;
@@ -159,9 +182,6 @@ template <class C, int S> __device__ __forceinline__ void tile_stage(Ctx<C>& x)
                         }
                     }
                 }
-            } else {
-#pragma unroll
-                for (int r = 0; r < R; ++r) v[b][r] = V{0, 0};
             }
         } else {
 #pragma unroll
@@ -209,19 +229,18 @@ template <class C, int S> __device__ __forceinline__ void tile_stage(Ctx<C>& x)
     }
 }
 
-template <class C, int S> __device__ __forceinline__ void stage_loop(Ctx<C>& x)
+template <class C, int S> __device__ __forceinline__ void stage_loop(Ctx<C>& x, FirstIn<C>& f)
 {
     if constexpr (S < C::NS) {
-        tile_stage<C, S>(x);
-        stage_loop<C, S + 1>(x);
+        tile_stage<C, S>(x, f);
+        stage_loop<C, S + 1>(x, f);
     }
 }
 
-// One tile (NB sequences) of the pass described by `a`; `tile` selects the sequences,
-// `by` the batch item.  Callable from any kernel whose workgroup has at least C::WG threads
-// and which provides C::LDS_ELEMS elements of LDS; extra threads only join the barriers.
+// Thread context for one tile (NB sequences) of the pass described by `a`; `tile` selects
+// the sequences, `by` the batch item.
 template <class C>
-__device__ __forceinline__ void tile_body(const TileArgs& a, unsigned tile, unsigned by, typename C::V* lds)
+__device__ __forceinline__ Ctx<C> make_ctx(const TileArgs& a, unsigned tile, unsigned by, typename C::V* lds)
 {
     using V = typename C::V;
     const int tid = threadIdx.x;
@@ -246,14 +265,20 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, unsigned tile, unsi
     x.out = reinterpret_cast<V*>(a.out) + (long long)by * a.bdist;
     x.tw = reinterpret_cast<const V*>(a.tw);
     x.base_in = x.valid ? (a.seq_off ? a.seq_off[x.tg] : x.tg * a.its) : 0;
+    return x;
+}
 
-    stage_loop<C, 0>(x);
-
+// Transposed store of a tile whose load and store modes differ: the LDS holds the
+// natural-order outputs of the NB sequences.
+template <class C> __device__ __forceinline__ void tile_finish(const Ctx<C>& x, unsigned tile)
+{
+    using V = typename C::V;
     if constexpr (!C::DIRECT) {
-        // transposed store: the LDS holds natural-order outputs of the NB sequences
         constexpr int PER = C::L / C::TPT;
+        const TileArgs& a = x.a;
         const long long ns = a.ns_pass;
         const double sc = a.scale;
+        const int tid = threadIdx.x;
         if (x.active) {
 #pragma unroll
             for (int i = 0; i < PER; ++i) {
@@ -268,7 +293,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, unsigned tile, unsi
                 }
                 const long long tgs = (long long)tile * C::NB + ts;
                 if (tgs < a.ntrans) {
-                    V u = lds[C::lidx(ts, q)];
+                    V u = x.lds[C::lidx(ts, q)];
                     if (a.conj_out) u.y = -u.y;
                     if (sc != 1.0) u = cscale(u, (decltype(u.x))sc);
                     const long long ob = a.seq_off ? a.seq_off[tgs] : (tgs / ns) * ns * C::L + (tgs % ns);
@@ -277,6 +302,19 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, unsigned tile, unsi
             }
         }
     }
+}
+
+// One tile, loaded and transformed.  Callable from any kernel whose workgroup has at least
+// C::WG threads and which provides C::LDS_ELEMS elements of LDS; extra threads only join
+// the barriers.
+template <class C>
+__device__ __forceinline__ void tile_body(const TileArgs& a, unsigned tile, unsigned by, typename C::V* lds)
+{
+    Ctx<C> x = make_ctx<C>(a, tile, by, lds);
+    FirstIn<C> f;
+    load_first<C>(x, f);
+    stage_loop<C, 0>(x, f);
+    tile_finish<C>(x, tile);
 }
 
 template <class C> __global__ __launch_bounds__(C::WG) void tile_kernel(TileArgs a)

@@ -197,6 +197,35 @@ def test_graph_replays_match_eager(torch_cuda, n, vt, tt):
             assert np.array_equal(host(s), x)
 
 
+def test_autotuner_measures_then_caches(torch_cuda, tmp_path, monkeypatch):
+    """A multi-pass plan times several pass splits on the device, records the winner in the
+    JSON cache, and the next plan of the same shape reuses it."""
+    import json
+
+    cache = tmp_path / "tune.json"
+    monkeypatch.setenv("GFFTX_TUNE_CACHE", str(cache))
+    n = 2 ** 20
+    x = inputs.complex_normal(n, 8).astype(np.complex64)
+    ref = oracle_fwd(x)
+    with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.IN_PLACE) as p:
+        d1 = json.loads(p.describe())
+        d = dev(torch_cuda, x)
+        p.fft(d)
+        assert rel_l2(host(d), ref) <= 2e-7
+    assert isinstance(d1["tuned"], list) and len(d1["tuned"]) >= 2
+    text = cache.read_text()
+    assert '"version": 1' in text and "n=1048576" in text
+    with api.Plan(n, api.COMPLEX_FLOAT, api.IDFT, api.OUT_OF_PLACE) as p:
+        d2 = json.loads(p.describe())
+        assert d2["tuned"] == "cache"
+        s = dev(torch_cuda, ref.astype(np.complex64))
+        o = torch_cuda.empty_like(s)
+        p.fft(s, o)
+        assert rel_l2(host(o), x) <= 4e-7
+    with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.IN_PLACE, flags=api.FLAG_NO_AUTOTUNE) as p:
+        assert "tuned" not in json.loads(p.describe())
+
+
 def test_out_of_place_alias_rejected(torch_cuda):
     with api.Plan(256, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE) as p:
         d = torch_cuda.zeros(256, dtype=torch_cuda.complex64, device="cuda")
