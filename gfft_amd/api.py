@@ -16,7 +16,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgfftx.so")
+LIB_PATH = os.environ.get("GFFTX_LIBRARY", os.path.join(HERE, "libgfftx.so"))
 
 # IDs as in GFFT (src/gfftpolicy.h:79,95,111,127 / :226,245,263,283 / :144,181)
 DOUBLE, FLOAT, COMPLEX_DOUBLE, COMPLEX_FLOAT = 0, 1, 2, 3

@@ -69,7 +69,7 @@ constexpr int kRaggedLds = ragged_lds_elems();
 // its own instead of as part of one very large switch.
 template <class CFG> __device__ __noinline__ void ragged_tile(const TileArgs& a, unsigned tile, float2* lds)
 {
-    tile_body<CFG>(a, tile, 0, lds);
+    tile_body<CFG, CFG::WG < RAGGED_WG>(a, tile, 0, lds);
 }
 
 __device__ __forceinline__ void run_cfg(int cfg, const TileArgs& a, unsigned tile, float2* lds)

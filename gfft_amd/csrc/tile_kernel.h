@@ -92,13 +92,14 @@ struct TileCfg {
 template <class C> struct Ctx {
     using V = typename C::V;
     V* lds;
-    const V* in;
-    V* out;
+    const V* in;         // input of this thread's sequence: element e at in[e * ies]
+    V* out;              // output of this thread's sequence: element q at out[q * ns]
     const V* tw;
     int t, jt;
     long long tg;
-    long long base_in;   // element offset of this thread's sequence (input side)
-    bool valid;          // sequence exists
+    unsigned ies;        // input element stride
+    unsigned ons;        // output element stride (the store rule's Ns)
+    bool valid;          // sequence exists (loads are clamped to a valid one regardless)
     bool active;         // thread takes part (threads beyond C::WG idle through the barriers)
     TileArgs a;
 };
@@ -119,35 +120,41 @@ __device__ __forceinline__ double2 pass_root(const TileArgs& a, long long m)
     return dmul(lo, hi);
 }
 
+// Conjugation as a sign-bit flip of the imaginary part: mask is 0 or the sign bit.
+__device__ __forceinline__ float2 flip_im(float2 u, unsigned mask)
+{
+    u.y = __uint_as_float(__float_as_uint(u.y) ^ mask);
+    return u;
+}
+__device__ __forceinline__ double2 flip_im(double2 u, unsigned mask)
+{
+    u.y = __longlong_as_double(__double_as_longlong(u.y) ^ ((long long)mask << 32));
+    return u;
+}
+
 // First-stage inputs of one thread (raw loads, conjugated for inverse transforms).
 template <class C> struct FirstIn {
     typename C::V v[C::nbf(0)][C::R[0]];
 };
 
-template <class C> __device__ __forceinline__ void load_first(const Ctx<C>& x, FirstIn<C>& f)
+template <class C, bool IDLE> __device__ __forceinline__ void load_first(const Ctx<C>& x, FirstIn<C>& f)
 {
     using V = typename C::V;
     constexpr int R = C::R[0];
     constexpr int B = C::nbf(0);
     constexpr int LR = C::L / R;
     constexpr bool GUARD = (B * C::TPT > LR);
+    const unsigned mask = x.a.conj_in ? 0x80000000u : 0u;
 #pragma unroll
     for (int b = 0; b < B; ++b) {
         const int j = x.jt + b * C::TPT;
-        if (!x.active || (GUARD && j >= LR)) continue;
+        if ((IDLE && !x.active) || (GUARD && j >= LR)) continue;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            V u = V{0, 0};
-            if (x.valid) {
-                u = x.in[x.base_in + (long long)(j + r * LR) * x.a.ies];
-                if (x.a.conj_in) u.y = -u.y;
-            }
-            f.v[b][r] = u;
-        }
+        for (int r = 0; r < R; ++r) f.v[b][r] = flip_im(x.in[(unsigned)(j + r * LR) * x.ies], mask);
     }
 }
 
-template <class C, int S> __device__ __forceinline__ void tile_stage(Ctx<C>& x, FirstIn<C>& f)
+template <class C, bool IDLE, int S> __device__ __forceinline__ void tile_stage(Ctx<C>& x, FirstIn<C>& f)
 {
     using V = typename C::V;
     constexpr int R = C::R[S];
@@ -162,24 +169,22 @@ template <class C, int S> __device__ __forceinline__ void tile_stage(Ctx<C>& x, 
 #pragma unroll
     for (int b = 0; b < B; ++b) {
         const int j = x.jt + b * C::TPT;
-        if (!x.active || (GUARD && j >= LR)) continue;
+        if ((IDLE && !x.active) || (GUARD && j >= LR)) continue;
         if constexpr (FIRST) {
             const TileArgs& a = x.a;
 #pragma unroll
             for (int r = 0; r < R; ++r) v[b][r] = f.v[b][r];
-            if (x.valid) {
-                if (a.ptw_lo != nullptr) {
-                    // W_M^{kp*e}, e = j + r*LR: base W^{kp*j} times step^r, step = W^{kp*LR}
-                    const long long kp = x.tg % a.ns_pass;
-                    if (kp != 0) {
-                        const double2 step = pass_root(a, kp * LR);
-                        double2 w = pass_root(a, kp * j);
-                        v[b][0] = cmul(v[b][0], to_v<V>(w));
+            if (a.ptw_lo != nullptr) {
+                // W_M^{kp*e}, e = j + r*LR: base W^{kp*j} times step^r, step = W^{kp*LR}
+                const long long kp = x.tg % a.ns_pass;
+                if (kp != 0) {
+                    const double2 step = pass_root(a, kp * LR);
+                    double2 w = pass_root(a, kp * j);
+                    v[b][0] = cmul(v[b][0], to_v<V>(w));
 #pragma unroll
-                        for (int r = 1; r < R; ++r) {
-                            w = dmul(w, step);
-                            v[b][r] = cmul(v[b][r], to_v<V>(w));
-                        }
+                    for (int r = 1; r < R; ++r) {
+                        w = dmul(w, step);
+                        v[b][r] = cmul(v[b][r], to_v<V>(w));
                     }
                 }
             }
@@ -197,21 +202,26 @@ template <class C, int S> __device__ __forceinline__ void tile_stage(Ctx<C>& x, 
     if constexpr (!FIRST) __syncthreads();
 
     if constexpr (LAST && C::DIRECT) {
-        if (x.valid && x.active) {
+        if (x.valid && (!IDLE || x.active)) {
             const TileArgs& a = x.a;
-            const long long ns = a.ns_pass;
-            const long long base = a.seq_off ? a.seq_off[x.tg] : (x.tg / ns) * ns * C::L + (x.tg % ns);
-            const double sc = a.scale;
+            const unsigned mask = a.conj_out ? 0x80000000u : 0u;
+            if (a.scale == 1.0) {
 #pragma unroll
-            for (int b = 0; b < B; ++b) {
-                const int j = x.jt + b * C::TPT;
-                if (GUARD && j >= LR) continue;
+                for (int b = 0; b < B; ++b) {
+                    const int j = x.jt + b * C::TPT;
+                    if (GUARD && j >= LR) continue;
 #pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    V u = v[b][r];
-                    if (a.conj_out) u.y = -u.y;
-                    if (sc != 1.0) u = cscale(u, (decltype(u.x))sc);
-                    x.out[base + (long long)(j + r * LR) * ns] = u;
+                    for (int r = 0; r < R; ++r) x.out[(unsigned)(j + r * LR) * x.ons] = flip_im(v[b][r], mask);
+                }
+            } else {
+                const auto sc = (decltype(v[0][0].x))a.scale;
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    const int j = x.jt + b * C::TPT;
+                    if (GUARD && j >= LR) continue;
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        x.out[(unsigned)(j + r * LR) * x.ons] = flip_im(cscale(v[b][r], sc), mask);
                 }
             }
         }
@@ -219,7 +229,7 @@ template <class C, int S> __device__ __forceinline__ void tile_stage(Ctx<C>& x, 
 #pragma unroll
         for (int b = 0; b < B; ++b) {
             const int j = x.jt + b * C::TPT;
-            if (!x.active || (GUARD && j >= LR)) continue;
+            if ((IDLE && !x.active) || (GUARD && j >= LR)) continue;
             const int k = j % NSS;
             const int base = (j / NSS) * NSS * R + k;
 #pragma unroll
@@ -229,17 +239,29 @@ template <class C, int S> __device__ __forceinline__ void tile_stage(Ctx<C>& x, 
     }
 }
 
-template <class C, int S> __device__ __forceinline__ void stage_loop(Ctx<C>& x, FirstIn<C>& f)
+template <class C, bool IDLE, int S> __device__ __forceinline__ void stage_loop(Ctx<C>& x, FirstIn<C>& f)
 {
     if constexpr (S < C::NS) {
-        tile_stage<C, S>(x, f);
-        stage_loop<C, S + 1>(x, f);
+        tile_stage<C, IDLE, S>(x, f);
+        stage_loop<C, IDLE, S + 1>(x, f);
     }
 }
 
+// Output element offset of sequence tg under the store rule with Ns = ns (relative to the
+// batch item): (tg/ns)*ns*L + tg%ns, or the sequence's own offset in the ragged layout.
+template <class C> __device__ __forceinline__ long long out_base(const TileArgs& a, long long tg)
+{
+    if (a.seq_off) return a.seq_off[tg];
+    const unsigned ns = (unsigned)a.ns_pass;
+    if (ns == 1) return tg * C::L;
+    const unsigned t32 = (unsigned)tg;  // sequences per batch item stay below 2^32
+    return (long long)(t32 / ns) * ns * C::L + (t32 % ns);
+}
+
 // Thread context for one tile (NB sequences) of the pass described by `a`; `tile` selects
-// the sequences, `by` the batch item.
-template <class C>
+// the sequences, `by` the batch item.  Loads of sequences past the end are clamped to the
+// last one (their results are never stored).
+template <class C, bool IDLE>
 __device__ __forceinline__ Ctx<C> make_ctx(const TileArgs& a, unsigned tile, unsigned by, typename C::V* lds)
 {
     using V = typename C::V;
@@ -247,7 +269,7 @@ __device__ __forceinline__ Ctx<C> make_ctx(const TileArgs& a, unsigned tile, uns
     Ctx<C> x;
     x.lds = lds;
     x.a = a;
-    x.active = tid < C::WG;
+    x.active = !IDLE || tid < C::WG;
     if (C::LOADT) {
         x.t = tid % C::NB;
         x.jt = tid / C::NB;
@@ -255,31 +277,38 @@ __device__ __forceinline__ Ctx<C> make_ctx(const TileArgs& a, unsigned tile, uns
         x.t = tid / C::TPT;
         x.jt = tid % C::TPT;
     }
-    if (!x.active) {
+    if (IDLE && !x.active) {
         x.t = 0;
         x.jt = 0;
     }
     x.tg = (long long)tile * C::NB + x.t;
     x.valid = x.tg < a.ntrans;
-    x.in = reinterpret_cast<const V*>(a.in) + (long long)by * a.bdist;
-    x.out = reinterpret_cast<V*>(a.out) + (long long)by * a.bdist;
+    const long long tl = x.valid ? x.tg : a.ntrans - 1;
+    const V* in = reinterpret_cast<const V*>(a.in) + (long long)by * a.bdist;
+    V* out = reinterpret_cast<V*>(a.out) + (long long)by * a.bdist;
+    x.in = in + (a.seq_off ? a.seq_off[tl] : tl * a.its);
+    x.out = out + out_base<C>(a, tl);
+    x.ies = (unsigned)a.ies;
+    x.ons = (unsigned)a.ns_pass;
     x.tw = reinterpret_cast<const V*>(a.tw);
-    x.base_in = x.valid ? (a.seq_off ? a.seq_off[x.tg] : x.tg * a.its) : 0;
     return x;
 }
 
 // Transposed store of a tile whose load and store modes differ: the LDS holds the
 // natural-order outputs of the NB sequences.
-template <class C> __device__ __forceinline__ void tile_finish(const Ctx<C>& x, unsigned tile)
+template <class C, bool IDLE> __device__ __forceinline__ void tile_finish(const Ctx<C>& x, unsigned tile, unsigned by)
 {
     using V = typename C::V;
     if constexpr (!C::DIRECT) {
         constexpr int PER = C::L / C::TPT;
         const TileArgs& a = x.a;
-        const long long ns = a.ns_pass;
-        const double sc = a.scale;
+        const unsigned ns = (unsigned)a.ns_pass;
+        const unsigned mask = a.conj_out ? 0x80000000u : 0u;
+        const auto sc = (decltype(V{}.x))a.scale;
+        const bool scale = a.scale != 1.0;
         const int tid = threadIdx.x;
-        if (x.active) {
+        V* out = reinterpret_cast<V*>(a.out) + (long long)by * a.bdist;
+        if (!IDLE || x.active) {
 #pragma unroll
             for (int i = 0; i < PER; ++i) {
                 const int lin = tid + i * C::WG;
@@ -294,10 +323,8 @@ template <class C> __device__ __forceinline__ void tile_finish(const Ctx<C>& x, 
                 const long long tgs = (long long)tile * C::NB + ts;
                 if (tgs < a.ntrans) {
                     V u = x.lds[C::lidx(ts, q)];
-                    if (a.conj_out) u.y = -u.y;
-                    if (sc != 1.0) u = cscale(u, (decltype(u.x))sc);
-                    const long long ob = a.seq_off ? a.seq_off[tgs] : (tgs / ns) * ns * C::L + (tgs % ns);
-                    x.out[ob + (long long)q * ns] = u;
+                    if (scale) u = cscale(u, sc);
+                    out[out_base<C>(a, tgs) + (unsigned)q * ns] = flip_im(u, mask);
                 }
             }
         }
@@ -305,23 +332,23 @@ template <class C> __device__ __forceinline__ void tile_finish(const Ctx<C>& x, 
 }
 
 // One tile, loaded and transformed.  Callable from any kernel whose workgroup has at least
-// C::WG threads and which provides C::LDS_ELEMS elements of LDS; extra threads only join
-// the barriers.
-template <class C>
+// C::WG threads (IDLE: more than C::WG, the extra ones only join the barriers) and which
+// provides C::LDS_ELEMS elements of LDS.
+template <class C, bool IDLE = false>
 __device__ __forceinline__ void tile_body(const TileArgs& a, unsigned tile, unsigned by, typename C::V* lds)
 {
-    Ctx<C> x = make_ctx<C>(a, tile, by, lds);
+    Ctx<C> x = make_ctx<C, IDLE>(a, tile, by, lds);
     FirstIn<C> f;
-    load_first<C>(x, f);
-    stage_loop<C, 0>(x, f);
-    tile_finish<C>(x, tile);
+    load_first<C, IDLE>(x, f);
+    stage_loop<C, IDLE, 0>(x, f);
+    tile_finish<C, IDLE>(x, tile, by);
 }
 
 template <class C> __global__ __launch_bounds__(C::WG) void tile_kernel(TileArgs a)
 {
     using V = typename C::V;
     __shared__ V lds[C::LDS_ELEMS];
-    tile_body<C>(a, tile_index(blockIdx.x, gridDim.x, a.xcd), blockIdx.y, lds);
+    tile_body<C, false>(a, tile_index(blockIdx.x, gridDim.x, a.xcd), blockIdx.y, lds);
 }
 
 }  // namespace gfftx
