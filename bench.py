@@ -13,7 +13,14 @@ Also reported on the same line:
                 sample, same input distribution;
   fp64          the same round trip in FP64.
 
+roofline.traffic is the HBM traffic of the dominant kernel per launch from two rocprofv3
+--pmc passes (FETCH_SIZE, WRITE_SIZE) over `python bench.py --counters`, each scaled by what
+the same counter reads for a device copy of 256 MiB in that run (FETCH_SIZE reads half of
+the bytes read, WRITE_SIZE all of the bytes written).  roofline.copy_GBps is a 1 GiB device
+copy timed in this run: the streaming rate HBM reaches in practice.
+
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1]
+       python bench.py --counters      (workloads only, for hardware-counter passes)
 """
 from __future__ import annotations
 
@@ -31,6 +38,82 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 
+# rocprofv3 --pmc over `python bench.py --counters` (N = 2^24 FP32 pass kernels): average
+# FETCH_SIZE and WRITE_SIZE per dispatch in KiB, and the same counters for a 256 MiB device
+# copy (__amd_rocclr_copyBuffer) in that run.
+PMC_COPY_KIB = {"fetch": 131088.56, "write": 262144.0, "bytes_kib": 262144.0}
+PMC_KIB = {
+    "tile_float_L256_nb16_wg256_truefalse": {"fetch": 65637.96, "write": 131072.0},
+    "tile_float_L256_nb16_wg256_truetrue": {"fetch": 65750.77, "write": 131072.0},
+}
+
+
+def hbm_traffic(kernel_name):
+    """Calibrated HBM bytes per launch of a 2^24 FP32 pass kernel, or None if not measured."""
+    c = PMC_KIB.get(kernel_name)
+    if c is None:
+        return None
+    fetch = c["fetch"] * PMC_COPY_KIB["bytes_kib"] / PMC_COPY_KIB["fetch"]
+    write = c["write"] * PMC_COPY_KIB["bytes_kib"] / PMC_COPY_KIB["write"]
+    return int(round((fetch + write) * 1024))
+
+
+def copy_rate(torch, nbytes=1 << 30, iters=20):
+    """Read+write bytes per second of a device-to-device copy of `nbytes`."""
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(iters):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) * 1e-3 / iters
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * nbytes / sec
+
+
+def counters_run():
+    """The workloads the counter passes read, a few calls each, after a 256 MiB copy."""
+    import torch
+
+    from gfft_amd import api, inputs
+
+    api.lib()
+    a = torch.empty(2 ** 26, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    print("step: 256 MiB copy", flush=True)
+    n = 2 ** 24
+    x = torch.from_numpy(inputs.complex_normal(n, 24).astype(np.complex64)).cuda()
+    with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.IN_PLACE) as p:
+        for _ in range(3):
+            p.fft(x)
+        torch.cuda.synchronize()
+        print("step: 2^24 FP32 forward x3", p.describe(), flush=True)
+    del x
+    xb = torch.randn(4096 * 16384, dtype=torch.complex64, device="cuda")
+    ob = torch.empty_like(xb)
+    for fl in (0, api.FLAG_SPLIT_MFMA):
+        with api.Plan(4096, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE, batch=16384, flags=fl) as p:
+            for _ in range(3):
+                p.fft(xb, ob)
+            torch.cuda.synchronize()
+    print("step: 16384 x 4096 FP32 plain and MFMA x3", flush=True)
+    del xb, ob
+    xd = torch.randn(1024 * 65536, dtype=torch.complex128, device="cuda")
+    with api.Plan(1024, api.DOUBLE, api.DFT, api.IN_PLACE, batch=65536) as p:
+        for _ in range(3):
+            p.fft(xd)
+        torch.cuda.synchronize()
+    print("step: 65536 x 1024 FP64 x3", flush=True)
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -41,20 +124,22 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-others", action="store_true")
+    ap.add_argument("--counters", action="store_true",
+                    help="run the profiled workloads only (for rocprofv3 --pmc passes)")
     return ap.parse_args()
 
 
 def round_trip_run(torch, api, n, dtype, steps, warmup, nbuf, seed=24):
-    """Times `steps` forward+inverse round trips; returns dict with timings and profile."""
+    """Times `steps` forward+inverse round trips (plans replayed as hipGraphs, no per-kernel
+    events inside the timed region), then measures per-kernel durations in a separate loop
+    with GFFTX_FLAG_PROFILE plans of the same split."""
     from gfft_amd import inputs
 
-    cdt = torch.complex64 if dtype == np.complex64 else torch.complex128
     vt = api.COMPLEX_FLOAT if dtype == np.complex64 else api.COMPLEX_DOUBLE
     x = inputs.complex_normal(n, seed).astype(dtype)
     bufs = [torch.from_numpy(x).to("cuda") for _ in range(nbuf)]
-    flags = api.FLAG_PROFILE
-    pf = api.Plan(n, vt, api.DFT, api.IN_PLACE, flags=flags)
-    pi = api.Plan(n, vt, api.IDFT, api.IN_PLACE, flags=flags)
+    pf = api.Plan(n, vt, api.DFT, api.IN_PLACE, flags=api.FLAG_GRAPH)
+    pi = api.Plan(n, vt, api.IDFT, api.IN_PLACE, flags=api.FLAG_GRAPH)
     stream = torch.cuda.current_stream()
     # one round trip: the identity up to rounding
     pf.fft(bufs[0])
@@ -62,13 +147,10 @@ def round_trip_run(torch, api, n, dtype, steps, warmup, nbuf, seed=24):
     torch.cuda.synchronize()
     rt_err = float(np.linalg.norm(bufs[0].cpu().numpy() - x) / np.linalg.norm(x))
     bufs[0].copy_(torch.from_numpy(x))
-    for i in range(warmup):
+    for i in range(max(warmup, nbuf)):
         b = bufs[i % nbuf]
         pf.fft(b)
         pi.fft(b)
-    torch.cuda.synchronize()
-    pf.profile()
-    pi.profile()
     torch.cuda.synchronize()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
@@ -81,10 +163,26 @@ def round_trip_run(torch, api, n, dtype, steps, warmup, nbuf, seed=24):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = start.elapsed_time(stop)
-    prof = {"fwd": pf.profile(), "inv": pi.profile()}
     desc = pf.describe()
     pf.close()
     pi.close()
+    # per-kernel durations (HIP events on the launch stream around every kernel)
+    qf = api.Plan(n, vt, api.DFT, api.IN_PLACE, flags=api.FLAG_PROFILE)
+    qi = api.Plan(n, vt, api.IDFT, api.IN_PLACE, flags=api.FLAG_PROFILE)
+    for i in range(3):
+        qf.fft(bufs[i % nbuf])
+        qi.fft(bufs[i % nbuf])
+    torch.cuda.synchronize()
+    qf.profile()
+    qi.profile()
+    for i in range(min(steps, 50)):
+        b = bufs[i % nbuf]
+        qf.fft(b)
+        qi.fft(b)
+    torch.cuda.synchronize()
+    prof = {"fwd": qf.profile(), "inv": qi.profile()}
+    qf.close()
+    qi.close()
     del bufs
     torch.cuda.empty_cache()
     return {"ms": ms, "wall_s": wall, "profile": prof, "rt_err": rt_err, "plan": desc}
@@ -209,6 +307,9 @@ def main():
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: every "
               "workload here runs on a single MI355X", file=sys.stderr)
         sys.exit(2)
+    if args.counters:
+        counters_run()
+        return
     import torch
 
     from gfft_amd import api
@@ -233,10 +334,13 @@ def main():
     dom_avg_s = dom_tot / dom_cnt * 1e-3
     bytes_per_launch = 2 * n * 8
     achieved = bytes_per_launch / dom_avg_s
+    copy_bps = copy_rate(torch)
     roofline = {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
+                "traffic": hbm_traffic(dom_key.split(":")[-1]) if args.log2n == 24 else None,
                 "kernel": dom_key, "avg_us": round(dom_avg_s * 1e6, 2),
-                "bytes_per_launch": bytes_per_launch}
+                "bytes_per_launch": bytes_per_launch,
+                "copy_GBps": round(copy_bps / 1e9, 1), "frac_of_copy": round(achieved / copy_bps, 4)}
     per_kernel = {k: {"avg_us": round(v[0] / v[1] * 1e3, 2),
                       "GBps_alg": round(bytes_per_launch / (v[0] / v[1] * 1e-3) / 1e9, 1)}
                   for k, v in kern.items()}
@@ -256,7 +360,9 @@ def main():
         "data": "synthetic: re,im ~ N(0,1) from SplitMix64 seed 24",
         "config": {"workload": f"C2C forward+inverse round trip, N=2^{args.log2n}, FP32, in place, "
                                f"multi-pass Stockham, {nbuf32} rotating buffers",
-                   "n": n, "passes": json.loads(r32["plan"])["passes"].__len__(),
+                   "n": n, "passes": len(json.loads(r32["plan"])["passes"]),
+                   "split": [p_["L"] for p_ in json.loads(r32["plan"])["passes"]],
+                   "launch": "hipGraph replay per direction",
                    "round_trip_relL2": r32["rt_err"]},
         "alg_GBps": round(2 * 2 * n * 8 / (ms_step * 1e-3) / 1e9, 1),
         "roofline": roofline,

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgfftx.so")
 
-SOURCES = ["kernels.hip", "ragged.hip", "plan.cpp"]
+SOURCES = ["kernels.hip", "ragged.hip", "bluestein.hip", "plan.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950",

@@ -81,6 +81,7 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int c = lane & 15, g = lane >> 4;
     float2 y[C::BPW][4];
+    const unsigned cmask_in = a.conj_in ? 0x80000000u : 0u;
 
 #pragma unroll
     for (int b = 0; b < C::BPW; ++b) {
@@ -88,19 +89,15 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
         const int t = blk / (C::COLS / 16);            // sequence within the workgroup
         const int j = (blk % (C::COLS / 16)) * 16 + c; // column
         const long long tg = t0 + t;
-        const bool valid = tg < a.ntrans;
         float2 v[4];
+        if constexpr (FIRST) {
+            // loads past the last sequence are clamped to it; their results are not stored
+            const float2* src = a.in + (tg < a.ntrans ? tg : a.ntrans - 1) * C::L;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int r = 4 * g + i;
-            const int e = j + r * C::COLS;
-            if constexpr (FIRST) {
-                float2 u = valid ? a.in[tg * C::L + e] : float2{0.f, 0.f};
-                if (a.conj_in) u.y = -u.y;
-                v[i] = u;
-            } else {
-                v[i] = lds[C::lidx(t, e)];
-            }
+            for (int i = 0; i < 4; ++i) v[i] = flip_im(src[(unsigned)(j + (4 * g + i) * C::COLS)], cmask_in);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v[i] = lds[C::lidx(t, j + (4 * g + i) * C::COLS)];
         }
         if constexpr (!FIRST) {
             const int k = j % NSS;
@@ -124,8 +121,9 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
             const _Float16 hr = (_Float16)xr, hi = (_Float16)xi;
             bh[2 * i] = hr;
             bh[2 * i + 1] = hi;
-            bl[2 * i] = (_Float16)(xr - (float)hr);
-            bl[2 * i + 1] = (_Float16)(xi - (float)hi);
+            // x - hi is exact in FP32; one mixed-precision FMA computes it from the FP16 value
+            bl[2 * i] = (_Float16)__builtin_fmaf((float)hr, -1.0f, xr);
+            bl[2 * i + 1] = (_Float16)__builtin_fmaf((float)hi, -1.0f, xi);
         }
         float4_t acc[2];
 #pragma unroll
@@ -142,6 +140,7 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
 
     if constexpr (!FIRST) __syncthreads();
 
+    const unsigned cmask_out = a.conj_out ? 0x80000000u : 0u;
 #pragma unroll
     for (int b = 0; b < C::BPW; ++b) {
         const int blk = wave * C::BPW + b;
@@ -149,21 +148,22 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
         const int j = (blk % (C::COLS / 16)) * 16 + c;
         const int k = j % NSS;
         const int base = (j / NSS) * NSS * 16 + k;
+        if constexpr (LAST) {
+            const long long tg = t0 + t;
+            if (tg < a.ntrans) {
+                float2* dst = a.out + tg * C::L;
+                const float sc = a.scale;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int q = 4 * g + i;
-            if constexpr (LAST) {
-                const long long tg = t0 + t;
-                if (tg < a.ntrans) {
+                for (int i = 0; i < 4; ++i) {
                     float2 u = y[b][i];
-                    if (a.conj_out) u.y = -u.y;
-                    u.x *= a.scale;
-                    u.y *= a.scale;
-                    a.out[tg * C::L + base + q * NSS] = u;
+                    u.x *= sc;
+                    u.y *= sc;
+                    dst[(unsigned)(base + (4 * g + i) * NSS)] = flip_im(u, cmask_out);
                 }
-            } else {
-                lds[C::lidx(t, base + q * NSS)] = y[b][i];
             }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) lds[C::lidx(t, base + (4 * g + i) * NSS)] = y[b][i];
         }
     }
     if constexpr (!LAST) __syncthreads();

@@ -222,6 +222,59 @@ bool choose_passes(int prec, unsigned long long n, std::vector<int>& out)
 
 }  // namespace
 
+// Instantiated hipGraphs keyed by the (in, out) device pointers they were captured with.  A
+// few entries let a caller rotate over several buffers without re-capturing; the least
+// recently used entry is replaced when all are taken.
+struct GraphCache {
+    static constexpr int kSlots = 8;
+    struct Entry {
+        const void* in = nullptr;
+        void* out = nullptr;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        unsigned long long used = 0;
+    };
+    Entry e[kSlots];
+    unsigned long long tick = 0;
+    hipStream_t cap_stream = nullptr;   // plan-owned stream the captures run on
+
+    Entry* find(const void* in, void* out)
+    {
+        for (auto& x : e)
+            if (x.exec && x.in == in && x.out == out) {
+                x.used = ++tick;
+                return &x;
+            }
+        return nullptr;
+    }
+    // frees the least recently used entry (or picks an empty one) and returns it
+    Entry* take()
+    {
+        Entry* v = &e[0];
+        for (auto& x : e) {
+            if (!x.exec) {
+                v = &x;
+                break;
+            }
+            if (x.used < v->used) v = &x;
+        }
+        release(*v);
+        return v;
+    }
+    static void release(Entry& x)
+    {
+        if (x.exec) (void)hipGraphExecDestroy(x.exec);
+        if (x.graph) (void)hipGraphDestroy(x.graph);
+        x = Entry{};
+    }
+    void clear()
+    {
+        for (auto& x : e) release(x);
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
+        cap_stream = nullptr;
+    }
+};
+
 struct gfftx_plan_s {
     int device = 0;
     unsigned long long n = 0;      // complex length of one transform
@@ -250,11 +303,14 @@ struct gfftx_plan_s {
     std::vector<Slot> slots;
     bool capturing = false;
     // GFFTX_FLAG_GRAPH: launches captured once per (in, out) pair and replayed
-    hipStream_t cap_stream = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    const void* graph_in = nullptr;
-    void* graph_out = nullptr;
+    GraphCache graphs;
+    // chirp-z path (n has a prime factor above 5): convolution of length blue_m
+    unsigned long long blue_m = 0;
+    gfftx_plan_s* blue_fwd = nullptr;  // forward C2C of length blue_m, in place on blue_work
+    gfftx_plan_s* blue_inv = nullptr;  // inverse C2C (1/blue_m), in place on blue_work
+    void* blue_chirp = nullptr;        // n entries W_{2n}^{j^2}
+    void* blue_filter = nullptr;       // blue_m entries: spectrum of the chirp filter
+    void* blue_work = nullptr;         // batch x blue_m
 };
 
 namespace {
@@ -553,8 +609,12 @@ void prof_end(gfftx_plan_s* p, size_t slot, hipStream_t s, const std::pair<hipEv
     p->slots[slot].pending.push_back(ev);
 }
 
-int run_passes(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
+// Runs the pass list of `p`.  Profiling events go to the slots of `prof` (the plan itself,
+// or the plan that owns `p` as an inner plan) starting at `slot0`.
+int run_passes(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, gfftx_plan_s* prof = nullptr,
+               size_t slot0 = 0)
 {
+    if (!prof) prof = p;
     const size_t m = p->passes.size();
     // buffer for the output of pass i (i < m-1): alternate so that no pass is in place and
     // the caller's input is never written unless it is also the output
@@ -581,9 +641,9 @@ int run_passes(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
             ma.scale = p->inverse ? (float)(1.0 / (double)p->n) : 1.0f;
             dim3 grid((unsigned)((ps.ntrans + ps.mfma->nb - 1) / ps.mfma->nb));
             std::pair<hipEvent_t, hipEvent_t> ev;
-            const bool prof = prof_begin(p, i, ps.mfma->name, s, &ev);
+            const bool on = prof_begin(prof, slot0 + i, ps.mfma->name, s, &ev);
             hipError_t e = ps.mfma->launch(ma, grid, s);
-            if (prof) prof_end(p, i, s, ev);
+            if (on) prof_end(prof, slot0 + i, s, ev);
             if (e != hipSuccess) return fail(GFFTX_ERR_LAUNCH, "%s: %s", ps.mfma->name, hipGetErrorString(e));
             src = dst[i];
             continue;
@@ -609,13 +669,66 @@ int run_passes(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
         const unsigned ntiles = (unsigned)((ps.ntrans + nb - 1) / nb);
         dim3 grid(ntiles, ps.batch);
         std::pair<hipEvent_t, hipEvent_t> ev;
-        const bool prof = prof_begin(p, i, ps.k->name, s, &ev);
+        const bool on = prof_begin(prof, slot0 + i, ps.k->name, s, &ev);
         hipError_t e = ps.k->launch(a, grid, s);
-        if (prof) prof_end(p, i, s, ev);
+        if (on) prof_end(prof, slot0 + i, s, ev);
         if (e != hipSuccess) return fail(GFFTX_ERR_LAUNCH, "%s: %s", ps.k->name, hipGetErrorString(e));
         src = dst[i];
     }
     return GFFTX_SUCCESS;
+}
+
+// Launch slots the complex transform of `p` uses (profiling); the split pass comes next.
+size_t c2c_slots(const gfftx_plan_s* p)
+{
+    if (!p->blue_fwd) return p->passes.size();
+    return 3 + p->blue_fwd->passes.size() + p->blue_inv->passes.size();
+}
+
+int blue_launch(gfftx_plan_s* p, int which, const BluesteinArgs& a, size_t slot, const char* name, hipStream_t s)
+{
+    std::pair<hipEvent_t, hipEvent_t> ev;
+    const bool on = prof_begin(p, slot, name, s, &ev);
+    hipError_t e = launch_bluestein(p->prec, which, a, s);
+    if (on) prof_end(p, slot, s, ev);
+    if (e != hipSuccess) return fail(GFFTX_ERR_LAUNCH, "%s: %s", name, hipGetErrorString(e));
+    return GFFTX_SUCCESS;
+}
+
+// Chirp-z transform: pre-multiply into the zero-padded work buffer, forward FFT of length m,
+// multiply by the filter spectrum, inverse FFT, post-multiply into `out`.
+int run_bluestein(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
+{
+    BluesteinArgs a;
+    memset(&a, 0, sizeof(a));
+    a.in = in;
+    a.out = out;
+    a.work = p->blue_work;
+    a.chirp = p->blue_chirp;
+    a.filter = p->blue_filter;
+    a.n = (long long)p->n;
+    a.m = (long long)p->blue_m;
+    a.dist = (long long)p->dist;
+    a.batch = (long long)p->batch;
+    a.conj = p->inverse ? 1 : 0;
+    a.scale = p->inverse ? 1.0 / (double)p->n : 1.0;
+    size_t slot = 0;
+    int rc = blue_launch(p, BLUESTEIN_PRE, a, slot++, "bluestein_pre", s);
+    if (rc) return rc;
+    rc = run_passes(p->blue_fwd, p->blue_work, p->blue_work, s, p, slot);
+    if (rc) return rc;
+    slot += p->blue_fwd->passes.size();
+    rc = blue_launch(p, BLUESTEIN_MUL, a, slot++, "bluestein_mul", s);
+    if (rc) return rc;
+    rc = run_passes(p->blue_inv, p->blue_work, p->blue_work, s, p, slot);
+    if (rc) return rc;
+    slot += p->blue_inv->passes.size();
+    return blue_launch(p, BLUESTEIN_POST, a, slot, "bluestein_post", s);
+}
+
+int run_c2c(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
+{
+    return p->blue_fwd ? run_bluestein(p, in, out, s) : run_passes(p, in, out, s);
 }
 
 int run_split(gfftx_plan_s* p, void* data, hipStream_t s)
@@ -629,7 +742,7 @@ int run_split(gfftx_plan_s* p, void* data, hipStream_t s)
     a.inverse = p->inverse ? 1 : 0;
     a.bins = (p->flags & GFFTX_FLAG_R2C_BINS) ? 1 : 0;
     std::pair<hipEvent_t, hipEvent_t> ev;
-    const size_t slot = p->passes.size();
+    const size_t slot = c2c_slots(p);
     const bool prof = prof_begin(p, slot, "split_kernel", s, &ev);
     hipError_t e = launch_split(p->prec, a, s);
     if (prof) prof_end(p, slot, s, ev);
@@ -642,10 +755,10 @@ int run_split(gfftx_plan_s* p, void* data, hipStream_t s)
 // unchanged), then the inverse complex FFT.
 int exec_device(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
 {
-    if (!p->real) return run_passes(p, in, out, s);
+    if (!p->real) return run_c2c(p, in, out, s);
     const size_t bytes = (size_t)p->dist * p->batch * elem_bytes(p->prec);
     if (!p->inverse) {
-        int rc = run_passes(p, in, out, s);
+        int rc = run_c2c(p, in, out, s);
         if (rc) return rc;
         return run_split(p, out, s);
     }
@@ -656,36 +769,39 @@ int exec_device(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
     }
     int rc = run_split(p, work, s);
     if (rc) return rc;
-    return run_passes(p, work, out, s);
+    return run_c2c(p, work, out, s);
 }
 
 // Replays the plan's launches as one captured hipGraph (captured again when the buffers
 // change).  Capture happens on a plan-owned stream; the graph is launched on the caller's.
 int exec_graph(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
 {
-    if (!p->graph_exec || p->graph_in != in || p->graph_out != out) {
-        if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
-        if (p->graph) (void)hipGraphDestroy(p->graph);
-        p->graph_exec = nullptr;
-        p->graph = nullptr;
-        if (!p->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
-        HIP_TRY(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+    GraphCache::Entry* g = p->graphs.find(in, out);
+    if (!g) {
+        if (!p->graphs.cap_stream) HIP_TRY(hipStreamCreateWithFlags(&p->graphs.cap_stream, hipStreamNonBlocking));
+        HIP_TRY(hipStreamBeginCapture(p->graphs.cap_stream, hipStreamCaptureModeThreadLocal));
         p->capturing = true;
-        const int rc = exec_device(p, in, out, p->cap_stream);
+        const int rc = exec_device(p, in, out, p->graphs.cap_stream);
         p->capturing = false;
         hipGraph_t gr = nullptr;
-        const hipError_t ce = hipStreamEndCapture(p->cap_stream, &gr);
+        const hipError_t ce = hipStreamEndCapture(p->graphs.cap_stream, &gr);
         if (rc) {
             if (gr) (void)hipGraphDestroy(gr);
             return rc;
         }
         if (ce != hipSuccess) return fail(GFFTX_ERR_DEVICE, "end capture: %s", hipGetErrorString(ce));
-        p->graph = gr;
-        HIP_TRY(hipGraphInstantiate(&p->graph_exec, gr, nullptr, nullptr, 0));
-        p->graph_in = in;
-        p->graph_out = out;
+        g = p->graphs.take();
+        g->graph = gr;
+        const hipError_t ie = hipGraphInstantiate(&g->exec, gr, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            GraphCache::release(*g);
+            return fail(GFFTX_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(ie));
+        }
+        g->in = in;
+        g->out = out;
+        g->used = ++p->graphs.tick;
     }
-    HIP_TRY(hipGraphLaunch(p->graph_exec, s));
+    HIP_TRY(hipGraphLaunch(g->exec, s));
     return GFFTX_SUCCESS;
 }
 
@@ -733,6 +849,122 @@ int exec_any(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
     return GFFTX_SUCCESS;
 }
 
+int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttype, int place, int device,
+                uint32_t flags);
+int create_plan_impl(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttype, int place, int device,
+                     uint32_t flags, const std::vector<int>& forced, const std::string& note);
+
+// Largest complex length the chirp-z path takes: its convolution length m < 2n must stay
+// within the multi-pass limit of the precision.
+unsigned long long bluestein_max_m(int prec) { return prec == PREC_F64 ? (1ull << 26) : (1ull << 28); }
+
+// Convolution length for the chirp-z path: the 2/3/5-smooth m >= 2n-1 (up to twice that)
+// with the lowest modelled cost, m times the sum over its passes of 1/bandwidth (a
+// single-pass tile counts as one pass at 4 TB/s).  0 when none can be planned.
+unsigned long long bluestein_length(int prec, unsigned long long n)
+{
+    const unsigned long long lo = 2 * n - 1, hi = 2 * lo, maxm = bluestein_max_m(prec);
+    std::vector<unsigned long long> cand;
+    for (unsigned long long a = 1; a <= hi; a *= 2)
+        for (unsigned long long b = a; b <= hi; b *= 3)
+            for (unsigned long long c = b; c <= hi; c *= 5)
+                if (c >= lo && c <= maxm) cand.push_back(c);
+    std::sort(cand.begin(), cand.end());
+    unsigned long long best = 0;
+    double best_cost = 0;
+    for (unsigned long long m : cand) {
+        double per = 0;
+        if (m <= (1ull << 30) && find_tile(prec, (int)m, false, false)) {
+            per = 1.0 / 4.0;
+        } else {
+            const std::vector<std::vector<int>> c = candidate_passes(prec, m, 1);
+            if (c.empty()) continue;
+            for (int L : c[0]) per += 1.0 / pass_bandwidth(prec, L);
+        }
+        const double cost = per * (double)m;
+        if (!best || cost < best_cost) {
+            best = m;
+            best_cost = cost;
+        }
+    }
+    return best;
+}
+
+// Moves the scratch buffers of `src` into `dst` where both have one (same length, batch and
+// precision): the two inner plans of a chirp-z plan run one after the other on one stream.
+void share_scratch(gfftx_plan_s* dst, gfftx_plan_s* src)
+{
+    for (int k = 0; k < 2; ++k) {
+        if (!dst->scratch[k] || !src->scratch[k]) continue;
+        auto it = std::find(dst->allocs.begin(), dst->allocs.end(), dst->scratch[k]);
+        if (it != dst->allocs.end()) dst->allocs.erase(it);
+        (void)hipFree(dst->scratch[k]);
+        dst->scratch[k] = src->scratch[k];
+    }
+}
+
+int build_bluestein(gfftx_plan_s* p)
+{
+    const unsigned long long n = p->n;
+    if (p->flags & GFFTX_FLAG_SPLIT_MFMA)
+        return fail(GFFTX_ERR_UNSUPPORTED, "split-precision MFMA path: FP32 C2C of length 256 or 4096 only (n=%llu)", n);
+    if (n < 2 || 2 * n - 1 > bluestein_max_m(p->prec))
+        return fail(GFFTX_ERR_UNSUPPORTED, "length %llu with a prime factor above 5: the chirp-z path takes 2 <= n <= %llu",
+                    n, (bluestein_max_m(p->prec) + 1) / 2);
+    const unsigned long long m = bluestein_length(p->prec, n);
+    if (!m) return fail(GFFTX_ERR_UNSUPPORTED, "no convolution length for n=%llu", n);
+    p->blue_m = m;
+    const int cvt = p->prec == PREC_F64 ? GFFTX_COMPLEX_DOUBLE : GFFTX_COMPLEX_FLOAT;
+    const uint32_t iflags = p->flags & GFFTX_FLAG_NO_AUTOTUNE;
+    gfftx_plan f = nullptr, b = nullptr;
+    int rc = create_plan(&f, m, p->batch, cvt, GFFTX_DFT, GFFTX_IN_PLACE, p->device, iflags);
+    if (rc) return rc;
+    p->blue_fwd = f;
+    rc = create_plan(&b, m, p->batch, cvt, GFFTX_IDFT, GFFTX_IN_PLACE, p->device, iflags);
+    if (rc) return rc;
+    p->blue_inv = b;
+    share_scratch(b, f);
+    const size_t eb = elem_bytes(p->prec);
+    rc = dev_alloc(p, &p->blue_work, (size_t)m * p->batch * eb);
+    if (rc) return rc;
+    rc = dev_alloc(p, &p->blue_chirp, (size_t)n * eb);
+    if (rc) return rc;
+    rc = dev_alloc(p, &p->blue_filter, (size_t)m * eb);
+    if (rc) return rc;
+    // chirp from the two-level root table of order 2n
+    ChirpTableArgs ca;
+    memset(&ca, 0, sizeof(ca));
+    ca.chirp = p->blue_chirp;
+    ca.n = (long long)n;
+    rc = upload_root_tables(p->allocs, (long long)(2 * n), const_cast<double2**>(&ca.lo),
+                            const_cast<double2**>(&ca.hi), &ca.shift);
+    if (rc) return rc;
+    hipError_t e = launch_chirp_table(p->prec, ca, nullptr);
+    if (e != hipSuccess) return fail(GFFTX_ERR_LAUNCH, "chirp table: %s", hipGetErrorString(e));
+    // filter spectrum: b in the work buffer, one forward transform of length m into the filter
+    BluesteinArgs ba;
+    memset(&ba, 0, sizeof(ba));
+    ba.work = p->blue_work;
+    ba.chirp = p->blue_chirp;
+    ba.n = (long long)n;
+    ba.m = (long long)m;
+    ba.batch = 1;
+    e = launch_bluestein(p->prec, BLUESTEIN_FILTER, ba, nullptr);
+    if (e != hipSuccess) return fail(GFFTX_ERR_LAUNCH, "chirp filter: %s", hipGetErrorString(e));
+    gfftx_plan fp = nullptr;
+    rc = create_plan_impl(&fp, m, 1, cvt, GFFTX_DFT, GFFTX_OUT_OF_PLACE, p->device, 0, {}, "");
+    if (rc) return rc;
+    rc = exec_device(fp, p->blue_work, p->blue_filter, nullptr);
+    const hipError_t se = hipStreamSynchronize(nullptr);
+    gfftx_plan_destroy(fp);
+    if (rc) return rc;
+    if (se != hipSuccess) return fail(GFFTX_ERR_DEVICE, "filter transform: %s", hipGetErrorString(se));
+    p->desc = "{\"n\":" + std::to_string(n) + ",\"batch\":" + std::to_string(p->batch) + ",\"precision\":\"" +
+              (p->prec == PREC_F64 ? "f64" : "f32") + "\",\"passes\":[],\"bluestein\":{\"m\":" + std::to_string(m) +
+              ",\"fwd\":" + f->desc + ",\"inv\":" + b->desc + "}}";
+    return GFFTX_SUCCESS;
+}
+
 int create_plan_impl(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttype, int place, int device,
                      uint32_t flags, const std::vector<int>& forced, const std::string& note)
 {
@@ -753,7 +985,6 @@ int create_plan_impl(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int
     }
     if (device < 0 || device >= ndev) return fail(GFFTX_ERR_DEVICE, "device %d out of range (%d)", device, ndev);
     const bool real = ttype == GFFTX_RDFT || ttype == GFFTX_IRDFT;
-    if (!only_235(n)) return fail(GFFTX_ERR_UNSUPPORTED, "length %llu has a prime factor above 5", (unsigned long long)n);
     if (real && (n % 2))
         return fail(GFFTX_ERR_UNSUPPORTED, "real transforms need an even complex length (n=%llu)",
                     (unsigned long long)n);
@@ -779,7 +1010,7 @@ int create_plan_impl(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int
     p->dist = n + ((real && (flags & GFFTX_FLAG_R2C_BINS)) ? 1 : 0);
     p->forced_lens = forced;
     p->tune_note = note;
-    int rc = build_c2c_passes(p);
+    int rc = only_235(n) ? build_c2c_passes(p) : build_bluestein(p);
     if (rc == GFFTX_SUCCESS && real) rc = upload_split_table(p);
     if (rc == GFFTX_SUCCESS && real && p->inverse)  // split pass of an out-of-place IRDFT works on a copy
         rc = dev_alloc(p, &p->real_work, (size_t)p->dist * batch * elem_bytes(p->prec));
@@ -789,8 +1020,7 @@ int create_plan_impl(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int
         if (rc == GFFTX_SUCCESS && p->passes.size() > 2) rc = dev_alloc(p, &p->scratch[1], bytes);
     }
     if (rc != GFFTX_SUCCESS) {
-        for (void* a : p->allocs) (void)hipFree(a);
-        delete p;
+        gfftx_plan_destroy(p);  // frees inner plans too; leaves the error message alone
         return rc;
     }
     *out = p;
@@ -811,11 +1041,8 @@ struct gfftx_ragged_s {
     RaggedArgs args{};
     int grid = 0;
     std::string desc;
-    // hipGraph replay (GFFTX_FLAG_GRAPH)
-    hipStream_t cap_stream = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    void* graph_data = nullptr;
+    // hipGraph replay (GFFTX_FLAG_GRAPH), keyed by the data pointer
+    GraphCache graphs;
     // host staging
     void* stage = nullptr;
     // profiling (GFFTX_FLAG_PROFILE): events around the launch
@@ -984,23 +1211,30 @@ int ragged_launch(gfftx_ragged_s* r, void* data, hipStream_t s)
         HIP_TRY(hipEventRecord(ev.first, s));
     }
     if (r->flags & GFFTX_FLAG_GRAPH) {
-        if (!r->graph_exec || r->graph_data != data) {
-            if (r->graph_exec) (void)hipGraphExecDestroy(r->graph_exec);
-            if (r->graph) (void)hipGraphDestroy(r->graph);
-            r->graph_exec = nullptr;
-            r->graph = nullptr;
-            if (!r->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&r->cap_stream, hipStreamNonBlocking));
-            HIP_TRY(hipStreamBeginCapture(r->cap_stream, hipStreamCaptureModeThreadLocal));
-            hipError_t le = launch_ragged(a, r->grid, r->cap_stream);
+        GraphCache::Entry* ge = r->graphs.find(data, data);
+        if (!ge) {
+            if (!r->graphs.cap_stream) HIP_TRY(hipStreamCreateWithFlags(&r->graphs.cap_stream, hipStreamNonBlocking));
+            HIP_TRY(hipStreamBeginCapture(r->graphs.cap_stream, hipStreamCaptureModeThreadLocal));
+            hipError_t le = launch_ragged(a, r->grid, r->graphs.cap_stream);
             hipGraph_t g = nullptr;
-            hipError_t ce = hipStreamEndCapture(r->cap_stream, &g);
-            if (le != hipSuccess) return fail(GFFTX_ERR_LAUNCH, "ragged capture: %s", hipGetErrorString(le));
+            hipError_t ce = hipStreamEndCapture(r->graphs.cap_stream, &g);
+            if (le != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return fail(GFFTX_ERR_LAUNCH, "ragged capture: %s", hipGetErrorString(le));
+            }
             if (ce != hipSuccess) return fail(GFFTX_ERR_DEVICE, "end capture: %s", hipGetErrorString(ce));
-            r->graph = g;
-            HIP_TRY(hipGraphInstantiate(&r->graph_exec, g, nullptr, nullptr, 0));
-            r->graph_data = data;
+            ge = r->graphs.take();
+            ge->graph = g;
+            const hipError_t ie = hipGraphInstantiate(&ge->exec, g, nullptr, nullptr, 0);
+            if (ie != hipSuccess) {
+                GraphCache::release(*ge);
+                return fail(GFFTX_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(ie));
+            }
+            ge->in = data;
+            ge->out = data;
+            ge->used = ++r->graphs.tick;
         }
-        HIP_TRY(hipGraphLaunch(r->graph_exec, s));
+        HIP_TRY(hipGraphLaunch(ge->exec, s));
     } else {
         hipError_t le = launch_ragged(a, r->grid, s);
         if (le != hipSuccess) return fail(GFFTX_ERR_LAUNCH, "ragged_kernel: %s", hipGetErrorString(le));
@@ -1263,9 +1497,9 @@ int gfftx_plan_destroy(gfftx_plan plan)
 {
     if (!plan) return GFFTX_SUCCESS;
     DeviceGuard g(plan->device);
-    if (plan->graph_exec) (void)hipGraphExecDestroy(plan->graph_exec);
-    if (plan->graph) (void)hipGraphDestroy(plan->graph);
-    if (plan->cap_stream) (void)hipStreamDestroy(plan->cap_stream);
+    plan->graphs.clear();
+    if (plan->blue_inv) gfftx_plan_destroy(plan->blue_inv);  // shares scratch owned by blue_fwd
+    if (plan->blue_fwd) gfftx_plan_destroy(plan->blue_fwd);
     for (auto& sl : plan->slots) {
         for (auto& ev : sl.pending) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
         for (auto& ev : sl.pool) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
@@ -1389,9 +1623,7 @@ int gfftx_ragged_destroy(gfftx_ragged plan)
 {
     if (!plan) return GFFTX_SUCCESS;
     DeviceGuard g(plan->device);
-    if (plan->graph_exec) (void)hipGraphExecDestroy(plan->graph_exec);
-    if (plan->graph) (void)hipGraphDestroy(plan->graph);
-    if (plan->cap_stream) (void)hipStreamDestroy(plan->cap_stream);
+    plan->graphs.clear();
     for (auto& ev : plan->pending) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
     for (auto& ev : plan->pool) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
     for (void* a : plan->allocs) (void)hipFree(a);

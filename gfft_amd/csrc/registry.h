@@ -51,6 +51,28 @@ struct SplitArgs {
 };
 hipError_t launch_split(int prec, const SplitArgs& a, hipStream_t s);
 
+// Chirp-z (Bluestein) path for lengths with a prime factor above 5 (bluestein.hip).
+struct ChirpTableArgs {
+    void* chirp;            // n entries c_j = W_{2n}^{j^2}, working precision
+    long long n;
+    const double2* lo;      // two-level FP64 root table of order 2n
+    const double2* hi;
+    int shift;
+};
+struct BluesteinArgs {
+    const void* in;         // caller's input (pre), batch items `dist` apart
+    void* out;              // caller's output (post)
+    void* work;             // batch x m convolution buffer
+    const void* chirp;      // n entries
+    const void* filter;     // m entries: FFT_m of the chirp filter
+    long long n, m, dist, batch;
+    int conj;               // inverse transform: conjugate on input and on output
+    double scale;           // post: 1 or 1/n
+};
+enum { BLUESTEIN_PRE = 0, BLUESTEIN_MUL = 1, BLUESTEIN_POST = 2, BLUESTEIN_FILTER = 3 };
+hipError_t launch_bluestein(int prec, int which, const BluesteinArgs& a, hipStream_t s);
+hipError_t launch_chirp_table(int prec, const ChirpTableArgs& a, hipStream_t s);
+
 // Digit reversal dst[rev(k)] = src[k] over radix^digits elements.
 hipError_t launch_digit_reverse(int prec, const void* src, void* dst, long long radix, int digits,
                                 long long n, hipStream_t s);

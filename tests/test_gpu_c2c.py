@@ -197,6 +197,29 @@ def test_graph_replays_match_eager(torch_cuda, n, vt, tt):
             assert np.array_equal(host(s), x)
 
 
+def test_graph_cache_rotating_buffers(torch_cuda):
+    """GFFTX_FLAG_GRAPH with more (in, out) pairs than cached graphs: every call, cached,
+    evicted or re-captured, equals the eager result for its own buffer."""
+    n = 3 * 2 ** 14
+    nbuf = 11
+    xs = [inputs.complex_normal(n, 100 + i).astype(np.complex64) for i in range(nbuf)]
+    with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE) as pe, \
+            api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE, flags=api.FLAG_GRAPH) as pg:
+        srcs = [dev(torch_cuda, x) for x in xs]
+        refs = []
+        for s_ in srcs:
+            o = torch_cuda.empty_like(s_)
+            pe.fft(s_, o)
+            refs.append(host(o))
+        outs = [torch_cuda.empty_like(s_) for s_ in srcs]
+        order = [0, 1, 2, 0, 1, 2] + list(range(nbuf)) + [10, 0, 5, 10]
+        for k, i in enumerate(order):
+            outs[i].zero_()
+            pg.fft(srcs[i], outs[i])
+            torch_cuda.cuda.synchronize()
+            assert np.array_equal(host(outs[i]), refs[i]), f"call {k} buffer {i}"
+
+
 def test_autotuner_measures_then_caches(torch_cuda, tmp_path, monkeypatch):
     """A multi-pass plan times several pass splits on the device, records the winner in the
     JSON cache, and the next plan of the same shape reuses it."""
@@ -236,7 +259,7 @@ def test_out_of_place_alias_rejected(torch_cuda):
 
 def test_unsupported_length_raises(torch_cuda):
     with pytest.raises(api.UnsupportedTransform):
-        api.Plan(7 * 64, api.COMPLEX_DOUBLE)
+        api.Plan(1, api.COMPLEX_DOUBLE)
 
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
