@@ -273,7 +273,45 @@ def other_workloads(torch, api):
         c5[label]["relL2_max_vs_fp64"] = float(errs.max())
     c5["mfma_over_plain"] = round(c5["plain"]["us"] / c5["mfma"]["us"], 3)
     res["c5_16384x4096_f32"] = c5
+    res["host_pointers"] = host_pointer_workloads(torch, api)
     return res
+
+
+def host_pointer_workloads(torch, api, reps=5):
+    """GFFT's fft(T*) on host arrays: the library's pinned, chunked pipeline against a plain
+    sequence of pageable copies (torch .cuda(), transform, copy back), median of `reps`.
+    Rate = host bytes in + out per second."""
+    from gfft_amd import inputs
+
+    out = {}
+    for label, n, batch in (("single_2p24_f32", 2 ** 24, 1), ("batch32_2p20_f32", 2 ** 20, 32)):
+        x = inputs.complex_normal(n * batch, 11).astype(np.complex64).reshape(batch, n)
+        y = np.zeros_like(x)
+        y2 = np.zeros_like(x)
+        y2_t = torch.from_numpy(y2)
+        with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE, batch=batch) as p:
+            p.fft(x, y)
+            d = torch.from_numpy(x).cuda()
+            o = torch.empty_like(d)
+            t_pipe, t_plain = [], []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                p.fft(x, y)
+                t_pipe.append(time.perf_counter() - t0)
+                t0 = time.perf_counter()
+                d.copy_(torch.from_numpy(x))
+                p.fft(d, o)
+                y2_t.copy_(o)
+                t_plain.append(time.perf_counter() - t0)
+        assert np.array_equal(y, y2)
+        nbytes = 2 * x.nbytes
+        tp, tq = float(np.median(t_pipe)), float(np.median(t_plain))
+        out[label] = {"pipelined_ms": round(tp * 1e3, 2), "pipelined_GBps": round(nbytes / tp / 1e9, 1),
+                      "pageable_copies_ms": round(tq * 1e3, 2), "pageable_GBps": round(nbytes / tq / 1e9, 1),
+                      "speedup": round(tq / tp, 2)}
+        del d, o
+    torch.cuda.empty_cache()
+    return out
 
 
 def cpu_baseline(n_log2: int):

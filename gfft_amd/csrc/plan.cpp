@@ -23,6 +23,7 @@
 #include <cctype>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/gfftx.h"
@@ -100,6 +101,12 @@ struct Pass {
     int ptw_shift = 0;
     long long ptw_m = 0;
     int xcd = 0;
+    bool items_in_ntrans = false;  // single-pass and MFMA: one sequence per batch item
+};
+
+// Batch items [b0, b0 + nb) of a plan; nb == 0 means every item.
+struct Range {
+    unsigned long long b0 = 0, nb = 0;
 };
 
 size_t elem_bytes(int prec) { return prec == PREC_F64 ? 16 : 8; }
@@ -292,7 +299,14 @@ struct gfftx_plan_s {
     void* scratch[2] = {nullptr, nullptr};
     void* split_tw = nullptr;
     void* real_work = nullptr;     // IRDFT out-of-place: split runs on a copy of src
-    DevBuf stage_in, stage_out;    // staging for host pointers
+    DevBuf stage_in, stage_out;    // device arrays a host-pointer call runs on
+    // host-pointer pipeline: two pinned buffers each way, a stream for each copy direction
+    void* pin_in[2] = {nullptr, nullptr};
+    void* pin_out[2] = {nullptr, nullptr};
+    size_t pin_bytes = 0;
+    hipStream_t h2d_stream = nullptr, d2h_stream = nullptr;
+    hipEvent_t pin_in_ev[2] = {nullptr, nullptr}, pin_out_ev[2] = {nullptr, nullptr};
+    hipEvent_t h2d_ev = nullptr, done_ev = nullptr;
     std::string desc;
     // GFFTX_FLAG_PROFILE: one (start, stop) event pair per launch slot and execution
     struct Slot {
@@ -460,6 +474,7 @@ int build_c2c_passes(gfftx_plan_s* p)
             return fail(GFFTX_ERR_UNSUPPORTED, "split-precision MFMA path: FP32 C2C of length 256 or 4096 only (n=%llu)", n);
         Pass ps;
         ps.mfma = me;
+        ps.items_in_ntrans = true;
         ps.ntrans = (long long)p->batch;
         const int rad[4] = {16, 16, 16, 16};
         int rc = upload_stage_table_r(p->allocs, PREC_F32, me->L == 256 ? 2 : 3, rad, &ps.stage_tw);
@@ -477,6 +492,7 @@ int build_c2c_passes(gfftx_plan_s* p)
         const TileEntry* k = single;
         Pass ps;
         ps.k = k;
+        ps.items_in_ntrans = true;
         ps.ntrans = (long long)p->batch;
         ps.its = (long long)p->dist;
         ps.ies = 1;
@@ -609,12 +625,16 @@ void prof_end(gfftx_plan_s* p, size_t slot, hipStream_t s, const std::pair<hipEv
     p->slots[slot].pending.push_back(ev);
 }
 
-// Runs the pass list of `p`.  Profiling events go to the slots of `prof` (the plan itself,
-// or the plan that owns `p` as an inner plan) starting at `slot0`.
-int run_passes(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, gfftx_plan_s* prof = nullptr,
+// Runs the pass list of `p` on batch items `r` of the arrays `in`/`out` (base pointers of
+// the whole batch).  Profiling events go to the slots of `prof` (the plan itself, or the
+// plan that owns `p` as an inner plan) starting at `slot0`.
+int run_passes(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, Range r, gfftx_plan_s* prof = nullptr,
                size_t slot0 = 0)
 {
     if (!prof) prof = p;
+    if (!r.nb) r.nb = p->batch;
+    const size_t off = (size_t)r.b0 * p->dist * elem_bytes(p->prec);
+    auto at = [off](const void* q) { return (void*)((char*)const_cast<void*>(q) + off); };
     const size_t m = p->passes.size();
     // buffer for the output of pass i (i < m-1): alternate so that no pass is in place and
     // the caller's input is never written unless it is also the output
@@ -631,9 +651,9 @@ int run_passes(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, gfftx_
         const Pass& ps = p->passes[i];
         if (ps.mfma) {
             MfmaArgs ma;
-            ma.in = (const float2*)src;
-            ma.out = (float2*)dst[i];
-            ma.ntrans = ps.ntrans;
+            ma.in = (const float2*)at(src);
+            ma.out = (float2*)at(dst[i]);
+            ma.ntrans = (long long)r.nb;
             ma.tw = (const float2*)ps.stage_tw;
             ma.amat = (const _Float16*)ps.amat;
             ma.conj_in = p->inverse ? 1 : 0;
@@ -650,9 +670,9 @@ int run_passes(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, gfftx_
         }
         TileArgs a;
         memset(&a, 0, sizeof(a));
-        a.in = src;
-        a.out = dst[i];
-        a.ntrans = ps.ntrans;
+        a.in = at(src);
+        a.out = at(dst[i]);
+        a.ntrans = ps.items_in_ntrans ? (long long)r.nb : ps.ntrans;
         a.its = ps.its;
         a.ies = ps.ies;
         a.ns_pass = ps.ns_pass;
@@ -666,8 +686,8 @@ int run_passes(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, gfftx_
         a.scale = (p->inverse && i == m - 1) ? 1.0 / (double)p->n : 1.0;
         a.xcd = ps.xcd;
         const long long nb = ps.k->nb;
-        const unsigned ntiles = (unsigned)((ps.ntrans + nb - 1) / nb);
-        dim3 grid(ntiles, ps.batch);
+        const unsigned ntiles = (unsigned)((a.ntrans + nb - 1) / nb);
+        dim3 grid(ntiles, ps.items_in_ntrans ? 1u : (unsigned)r.nb);
         std::pair<hipEvent_t, hipEvent_t> ev;
         const bool on = prof_begin(prof, slot0 + i, ps.k->name, s, &ev);
         hipError_t e = ps.k->launch(a, grid, s);
@@ -697,47 +717,50 @@ int blue_launch(gfftx_plan_s* p, int which, const BluesteinArgs& a, size_t slot,
 
 // Chirp-z transform: pre-multiply into the zero-padded work buffer, forward FFT of length m,
 // multiply by the filter spectrum, inverse FFT, post-multiply into `out`.
-int run_bluestein(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
+int run_bluestein(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, Range r)
 {
+    if (!r.nb) r.nb = p->batch;
+    const size_t eb = elem_bytes(p->prec);
     BluesteinArgs a;
     memset(&a, 0, sizeof(a));
-    a.in = in;
-    a.out = out;
-    a.work = p->blue_work;
+    a.in = (const char*)in + (size_t)r.b0 * p->dist * eb;
+    a.out = (char*)out + (size_t)r.b0 * p->dist * eb;
+    a.work = (char*)p->blue_work + (size_t)r.b0 * p->blue_m * eb;
     a.chirp = p->blue_chirp;
     a.filter = p->blue_filter;
     a.n = (long long)p->n;
     a.m = (long long)p->blue_m;
     a.dist = (long long)p->dist;
-    a.batch = (long long)p->batch;
+    a.batch = (long long)r.nb;
     a.conj = p->inverse ? 1 : 0;
     a.scale = p->inverse ? 1.0 / (double)p->n : 1.0;
     size_t slot = 0;
     int rc = blue_launch(p, BLUESTEIN_PRE, a, slot++, "bluestein_pre", s);
     if (rc) return rc;
-    rc = run_passes(p->blue_fwd, p->blue_work, p->blue_work, s, p, slot);
+    rc = run_passes(p->blue_fwd, p->blue_work, p->blue_work, s, r, p, slot);
     if (rc) return rc;
     slot += p->blue_fwd->passes.size();
     rc = blue_launch(p, BLUESTEIN_MUL, a, slot++, "bluestein_mul", s);
     if (rc) return rc;
-    rc = run_passes(p->blue_inv, p->blue_work, p->blue_work, s, p, slot);
+    rc = run_passes(p->blue_inv, p->blue_work, p->blue_work, s, r, p, slot);
     if (rc) return rc;
     slot += p->blue_inv->passes.size();
     return blue_launch(p, BLUESTEIN_POST, a, slot, "bluestein_post", s);
 }
 
-int run_c2c(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
+int run_c2c(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, Range r)
 {
-    return p->blue_fwd ? run_bluestein(p, in, out, s) : run_passes(p, in, out, s);
+    return p->blue_fwd ? run_bluestein(p, in, out, s, r) : run_passes(p, in, out, s, r);
 }
 
-int run_split(gfftx_plan_s* p, void* data, hipStream_t s)
+int run_split(gfftx_plan_s* p, void* data, hipStream_t s, Range r)
 {
+    if (!r.nb) r.nb = p->batch;
     SplitArgs a;
-    a.data = data;
+    a.data = (char*)data + (size_t)r.b0 * p->dist * elem_bytes(p->prec);
     a.n = (long long)p->n;
     a.bdist = (long long)p->dist;
-    a.nbatch = (long long)p->batch;
+    a.nbatch = (long long)r.nb;
     a.tw = p->split_tw;
     a.inverse = p->inverse ? 1 : 0;
     a.bins = (p->flags & GFFTX_FLAG_R2C_BINS) ? 1 : 0;
@@ -753,27 +776,30 @@ int run_split(gfftx_plan_s* p, void* data, hipStream_t s)
 // Full transform on device pointers.  Real forward: complex FFT then split on the output.
 // Real inverse: split on a copy of the input (the input of an out-of-place call is left
 // unchanged), then the inverse complex FFT.
-int exec_device(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
+// `r` selects batch items (all by default); in and out are the base pointers of the batch.
+int exec_device(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, Range r = Range())
 {
-    if (!p->real) return run_c2c(p, in, out, s);
-    const size_t bytes = (size_t)p->dist * p->batch * elem_bytes(p->prec);
+    if (!r.nb) r.nb = p->batch;
+    if (!p->real) return run_c2c(p, in, out, s, r);
     if (!p->inverse) {
-        int rc = run_c2c(p, in, out, s);
+        int rc = run_c2c(p, in, out, s, r);
         if (rc) return rc;
-        return run_split(p, out, s);
+        return run_split(p, out, s, r);
     }
     void* work = out;
     if (in != out) {
         work = p->real_work;
-        HIP_TRY(hipMemcpyAsync(work, in, bytes, hipMemcpyDeviceToDevice, s));
+        const size_t eb = elem_bytes(p->prec), off = (size_t)r.b0 * p->dist * eb;
+        HIP_TRY(hipMemcpyAsync((char*)work + off, (const char*)in + off, (size_t)p->dist * r.nb * eb,
+                               hipMemcpyDeviceToDevice, s));
     }
-    int rc = run_split(p, work, s);
+    int rc = run_split(p, work, s, r);
     if (rc) return rc;
-    return run_c2c(p, work, out, s);
+    return run_c2c(p, work, out, s, r);
 }
 
-// Replays the plan's launches as one captured hipGraph (captured again when the buffers
-// change).  Capture happens on a plan-owned stream; the graph is launched on the caller's.
+// Replays the plan's launches as one captured hipGraph, one per (in, out) pair (the last
+// eight kept).  Capture happens on a plan-owned stream; the graph is launched on the caller's.
 int exec_graph(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
 {
     GraphCache::Entry* g = p->graphs.find(in, out);
@@ -805,6 +831,148 @@ int exec_graph(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
     return GFFTX_SUCCESS;
 }
 
+// Copy between pageable and pinned host memory with up to 8 threads (chunks of 1 MiB or more).
+void host_copy(void* dst, const void* src, size_t len)
+{
+    unsigned t = std::thread::hardware_concurrency();
+    t = std::min<unsigned>(std::max(t, 1u), 8u);
+    t = (unsigned)std::min<size_t>(t, len >> 20);
+    if (t <= 1) {
+        memcpy(dst, src, len);
+        return;
+    }
+    const size_t part = ((len + t - 1) / t + 63) & ~(size_t)63;
+    std::vector<std::thread> th;
+    for (unsigned i = 1; i < t && i * part < len; ++i) {
+        const size_t o = i * part, l = std::min(part, len - o);
+        th.emplace_back([=] { memcpy((char*)dst + o, (const char*)src + o, l); });
+    }
+    memcpy(dst, src, std::min(part, len));
+    for (auto& x : th) x.join();
+}
+
+int ensure_host_pipeline(gfftx_plan_s* p, size_t want)
+{
+    if (p->pin_bytes >= want) return GFFTX_SUCCESS;
+    for (int k = 0; k < 2; ++k) {
+        if (p->pin_in[k]) (void)hipHostFree(p->pin_in[k]);
+        if (p->pin_out[k]) (void)hipHostFree(p->pin_out[k]);
+        p->pin_in[k] = p->pin_out[k] = nullptr;
+    }
+    p->pin_bytes = 0;
+    for (int k = 0; k < 2; ++k) {
+        if (hipHostMalloc(&p->pin_in[k], want, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc(&p->pin_out[k], want, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(GFFTX_ERR_ALLOC, "pinned staging of %zu bytes failed", want);
+        }
+    }
+    p->pin_bytes = want;
+    if (!p->h2d_stream) HIP_TRY(hipStreamCreateWithFlags(&p->h2d_stream, hipStreamNonBlocking));
+    if (!p->d2h_stream) HIP_TRY(hipStreamCreateWithFlags(&p->d2h_stream, hipStreamNonBlocking));
+    for (int k = 0; k < 2; ++k) {
+        if (!p->pin_in_ev[k]) HIP_TRY(hipEventCreateWithFlags(&p->pin_in_ev[k], hipEventDisableTiming));
+        if (!p->pin_out_ev[k]) HIP_TRY(hipEventCreateWithFlags(&p->pin_out_ev[k], hipEventDisableTiming));
+    }
+    if (!p->h2d_ev) HIP_TRY(hipEventCreateWithFlags(&p->h2d_ev, hipEventDisableTiming));
+    if (!p->done_ev) HIP_TRY(hipEventCreateWithFlags(&p->done_ev, hipEventDisableTiming));
+    return GFFTX_SUCCESS;
+}
+
+// Host-pointer call.  Batch items go in groups of about one staging chunk; for each group:
+// host -> pinned (threads) -> device on the upload stream, the transform of those items on
+// the caller's stream, device -> pinned on the download stream -> host.  Two pinned buffers
+// each way let the host copies overlap the DMA, and the two directions run at once.  A single
+// transform is one group: its upload and download are still chunked and overlapped with the
+// host copies.  Returns when `out` holds the result, as GFFT's fft() does.
+int exec_host(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, bool din, bool dout, size_t in_bytes,
+              size_t out_bytes)
+{
+    const size_t eb = elem_bytes(p->prec), full = (size_t)p->dist * p->batch * eb, item = (size_t)p->dist * eb;
+    if (!din && !p->stage_in.p) {
+        int rc = dev_alloc(p, &p->stage_in.p, full);
+        if (rc) return rc;
+        p->stage_in.bytes = full;
+    }
+    void* dev_in = din ? const_cast<void*>(in) : p->stage_in.p;
+    void* dev_out = dev_in;
+    if (in != out) {
+        if (dout) {
+            dev_out = out;
+        } else {
+            if (!p->stage_out.p) {
+                int rc = dev_alloc(p, &p->stage_out.p, full);
+                if (rc) return rc;
+                p->stage_out.bytes = full;
+            }
+            dev_out = p->stage_out.p;
+        }
+    }
+    const size_t chunk = (size_t)std::max(1, env_int("GFFTX_STAGE_CHUNK_MB", 16)) << 20;
+    int rc = ensure_host_pipeline(p, std::min(chunk, std::max(in_bytes, out_bytes)));
+    if (rc) return rc;
+    const size_t pb = p->pin_bytes;
+    const unsigned long long per = std::max<unsigned long long>(1, chunk / item);
+    struct Pending {
+        bool on = false;
+        size_t off = 0, len = 0;
+    } pend[2];
+    unsigned ki = 0, ko = 0;
+    auto retire = [&](int k) -> int {
+        if (!pend[k].on) return GFFTX_SUCCESS;
+        HIP_TRY(hipEventSynchronize(p->pin_out_ev[k]));
+        host_copy((char*)out + pend[k].off, p->pin_out[k], pend[k].len);
+        pend[k].on = false;
+        return GFFTX_SUCCESS;
+    };
+    for (unsigned long long b0 = 0; b0 < p->batch; b0 += per) {
+        const unsigned long long nb = std::min<unsigned long long>(per, p->batch - b0);
+        if (!din) {
+            const size_t lo = std::min<size_t>(b0 * item, in_bytes), hi = std::min<size_t>((b0 + nb) * item, in_bytes);
+            for (size_t o = lo; o < hi; o += pb) {
+                const size_t len = std::min(pb, hi - o);
+                const int k = ki++ & 1;
+                HIP_TRY(hipEventSynchronize(p->pin_in_ev[k]));  // its previous upload is done
+                host_copy(p->pin_in[k], (const char*)in + o, len);
+                HIP_TRY(hipMemcpyAsync((char*)dev_in + o, p->pin_in[k], len, hipMemcpyHostToDevice, p->h2d_stream));
+                HIP_TRY(hipEventRecord(p->pin_in_ev[k], p->h2d_stream));
+            }
+            HIP_TRY(hipEventRecord(p->h2d_ev, p->h2d_stream));
+            HIP_TRY(hipStreamWaitEvent(s, p->h2d_ev, 0));
+        }
+        rc = exec_device(p, dev_in, dev_out, s, Range{b0, nb});
+        if (rc) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamSynchronize(p->h2d_stream);
+            return rc;
+        }
+        if (!dout) {
+            HIP_TRY(hipEventRecord(p->done_ev, s));
+            HIP_TRY(hipStreamWaitEvent(p->d2h_stream, p->done_ev, 0));
+            const size_t lo = std::min<size_t>(b0 * item, out_bytes), hi = std::min<size_t>((b0 + nb) * item, out_bytes);
+            for (size_t o = lo; o < hi; o += pb) {
+                const size_t len = std::min(pb, hi - o);
+                const int k = ko++ & 1;
+                rc = retire(k);
+                if (rc) return rc;
+                HIP_TRY(hipMemcpyAsync(p->pin_out[k], (const char*)dev_out + o, len, hipMemcpyDeviceToHost,
+                                       p->d2h_stream));
+                HIP_TRY(hipEventRecord(p->pin_out_ev[k], p->d2h_stream));
+                pend[k].on = true;
+                pend[k].off = o;
+                pend[k].len = len;
+            }
+        }
+    }
+    rc = retire(ko & 1);  // the older of the two pending downloads first
+    if (rc) return rc;
+    rc = retire((ko + 1) & 1);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipStreamSynchronize(p->h2d_stream));
+    return GFFTX_SUCCESS;
+}
+
 int exec_any(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
 {
     DeviceGuard g(p->device);
@@ -821,32 +989,7 @@ int exec_any(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
         else in_bytes = real_side * eb;
     }
     if (din && dout) return exec_device(p, in, out, s);
-    // host staging: synchronous, as GFFT's fft() is
-    if (!p->stage_in.p) {
-        int rc = dev_alloc(p, &p->stage_in.p, full * eb);
-        if (rc) return rc;
-        p->stage_in.bytes = full * eb;
-    }
-    void* dev_in = din ? const_cast<void*>(in) : p->stage_in.p;
-    void* dev_out;
-    if (in == out) {
-        dev_out = dev_in;
-    } else if (dout) {
-        dev_out = out;
-    } else {
-        if (!p->stage_out.p) {
-            int rc = dev_alloc(p, &p->stage_out.p, full * eb);
-            if (rc) return rc;
-            p->stage_out.bytes = full * eb;
-        }
-        dev_out = p->stage_out.p;
-    }
-    if (!din) HIP_TRY(hipMemcpyAsync(dev_in, in, in_bytes, hipMemcpyHostToDevice, s));
-    int rc = exec_device(p, dev_in, dev_out, s);
-    if (rc) return rc;
-    if (!dout) HIP_TRY(hipMemcpyAsync(out, dev_out, out_bytes, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    return GFFTX_SUCCESS;
+    return exec_host(p, in, out, s, din, dout, in_bytes, out_bytes);
 }
 
 int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttype, int place, int device,
@@ -1498,6 +1641,16 @@ int gfftx_plan_destroy(gfftx_plan plan)
     if (!plan) return GFFTX_SUCCESS;
     DeviceGuard g(plan->device);
     plan->graphs.clear();
+    for (int k = 0; k < 2; ++k) {
+        if (plan->pin_in[k]) (void)hipHostFree(plan->pin_in[k]);
+        if (plan->pin_out[k]) (void)hipHostFree(plan->pin_out[k]);
+        if (plan->pin_in_ev[k]) (void)hipEventDestroy(plan->pin_in_ev[k]);
+        if (plan->pin_out_ev[k]) (void)hipEventDestroy(plan->pin_out_ev[k]);
+    }
+    if (plan->h2d_ev) (void)hipEventDestroy(plan->h2d_ev);
+    if (plan->done_ev) (void)hipEventDestroy(plan->done_ev);
+    if (plan->h2d_stream) (void)hipStreamDestroy(plan->h2d_stream);
+    if (plan->d2h_stream) (void)hipStreamDestroy(plan->d2h_stream);
     if (plan->blue_inv) gfftx_plan_destroy(plan->blue_inv);  // shares scratch owned by blue_fwd
     if (plan->blue_fwd) gfftx_plan_destroy(plan->blue_fwd);
     for (auto& sl : plan->slots) {

@@ -22,9 +22,18 @@
  *     GFFTX_FLAG_R2C_BINS selects the n+1-bin layout instead (the buffer then holds
  *     2n+2 scalars on the complex side).
  *
+ * Lengths: any n >= 2 (GFFT's out-of-place path takes any N, src/gfftalg.h:459-507).
+ * Products of 2, 3 and 5 run as Stockham passes (FP32 up to 2^28, FP64 up to 2^26); any
+ * other n runs as a chirp-z convolution of a 2/3/5-smooth length m >= 2n-1 with the same
+ * limits on m.  RDFT/IRDFT need an even n (GFFT's Separate assumes it,
+ * src/gfftswap.h:255,279).
+ *
  * Pointers may be device memory (the call is enqueued on `stream` and returns) or host
- * memory (the call stages through device buffers and returns after the result is back,
- * as GFFT's synchronous fft() requires).  `stream` is a hipStream_t; NULL is the null stream.
+ * memory (the call copies through pinned staging buffers in chunks, overlapping the host
+ * copies, both DMA directions and the transform of earlier batch items, and returns after
+ * the result is back, as GFFT's synchronous fft() requires; GFFTX_STAGE_CHUNK_MB sets the
+ * chunk, default 16).  `stream` is a hipStream_t; NULL is the null stream.  Calls that
+ * share a plan must not overlap: a plan owns its scratch and staging buffers.
  * No function throws; each returns a gfftx_status.  gfftx_last_error() gives detail for the
  * calling thread.
  */

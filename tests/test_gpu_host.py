@@ -1,0 +1,84 @@
+"""Host-pointer calls: the pinned, chunked pipeline behind GFFT's synchronous fft(T*).
+
+A host call runs the same kernels as a device call, item group by item group, so its result
+is bit-identical to the device call's.  The staging chunk is set to 1 MiB here
+(GFFTX_STAGE_CHUNK_MB) so that every case spans several groups or several chunks per item.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from gfft_amd import api, inputs
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def small_chunks(monkeypatch):
+    monkeypatch.setenv("GFFTX_STAGE_CHUNK_MB", "1")
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    import torch
+
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+CASES = [
+    # n, batch, value type, transform, place
+    (2 ** 16, 7, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE),     # 2 items per 1 MiB group
+    (2 ** 20, 3, api.COMPLEX_FLOAT, api.IDFT, api.IN_PLACE),        # 8 chunks per item, multi-pass
+    (3 * 2 ** 12, 40, api.COMPLEX_DOUBLE, api.DFT, api.IN_PLACE),   # ragged last group
+    (2 ** 15, 9, api.DOUBLE, api.RDFT, api.OUT_OF_PLACE),          # real forward
+    (2 ** 15, 9, api.FLOAT, api.IRDFT, api.OUT_OF_PLACE),          # real inverse (copy of src)
+    (1009, 300, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE),      # chirp-z, many groups
+    (5 * 2 ** 17, 2, api.COMPLEX_DOUBLE, api.DFT, api.OUT_OF_PLACE),
+]
+
+
+@pytest.mark.parametrize("n,batch,vt,tt,place", CASES)
+def test_host_call_equals_device_call(torch_cuda, n, batch, vt, tt, place):
+    dt = np.complex128 if vt in (api.DOUBLE, api.COMPLEX_DOUBLE) else np.complex64
+    x = inputs.complex_normal(n * batch, 600 + n).astype(dt).reshape(batch, n)
+    print(f"case host n={n} batch={batch} vt={vt} tt={tt} place={place}", flush=True)
+    with api.Plan(n, vt, tt, place, batch=batch) as p:
+        d = dev(torch_cuda, x)
+        if place == api.IN_PLACE:
+            p.fft(d)
+            ref = host(d)
+            h = x.copy()
+            p.fft(h)
+            assert np.array_equal(h, ref)
+        else:
+            o = torch_cuda.empty_like(d)
+            p.fft(d, o)
+            ref = host(o)
+            ho = np.zeros_like(x)
+            p.fft(x, ho)
+            assert np.array_equal(ho, ref)
+            # one side on the device, the other on the host
+            ho2 = np.zeros_like(x)
+            p.fft(d, ho2)
+            assert np.array_equal(ho2, ref)
+            o2 = torch_cuda.zeros_like(d)
+            p.fft(x, o2)
+            assert np.array_equal(host(o2), ref)
+            assert np.array_equal(host(d), x)
+
+
+def test_host_call_matches_oracle_and_repeats(torch_cuda):
+    """Repeated host calls on one plan (staging reused) stay equal to the oracle."""
+    n, batch = 2 ** 14, 20
+    x = inputs.complex_normal(n * batch, 3).reshape(batch, n)
+    ref = np.stack([O.ref_fft(r, 1) for r in x])
+    with api.Plan(n, api.COMPLEX_DOUBLE, api.DFT, api.OUT_OF_PLACE, batch=batch) as p:
+        for _ in range(3):
+            out = np.zeros_like(x)
+            p.fft(x, out)
+            assert rel_l2(out, ref) <= 1e-13
