@@ -91,6 +91,8 @@ template <class C> static hipError_t launch_tile(const TileArgs& a, dim3 grid, h
 #define GFFTX_PASS_SET(PREC, TYPE, NB4096, WG4096, NB1024, WG1024, NB1536, WG1536)         \
     GFFTX_TILE(PREC, TYPE, 256, 16, 256, true, false, 16, 16),                             \
     GFFTX_TILE(PREC, TYPE, 256, 16, 256, true, true, 16, 16),                              \
+    GFFTX_TILE(PREC, TYPE, 128, 32, 256, true, false, 16, 8),                              \
+    GFFTX_TILE(PREC, TYPE, 128, 32, 256, true, true, 16, 8),                               \
     GFFTX_TILE(PREC, TYPE, 1024, NB1024, WG1024, true, false, 16, 16, 4),                  \
     GFFTX_TILE(PREC, TYPE, 1024, NB1024, WG1024, true, true, 16, 16, 4),                   \
     GFFTX_TILE(PREC, TYPE, 1536, NB1536, WG1536, true, false, 16, 16, 2, 3),               \
@@ -113,6 +115,18 @@ TileEntry kTiles[] = {
     GFFTX_TILE(PREC_F32, float, 4096, 1, 256, true, true, 16, 16, 16),
     GFFTX_TILE(PREC_F32, float, 256, 32, 512, true, false, 16, 16),
     GFFTX_TILE(PREC_F32, float, 256, 32, 512, true, true, 16, 16),
+    GFFTX_TILE(PREC_F32, float, 256, 16, 128, true, false, 16, 16),
+    GFFTX_TILE(PREC_F32, float, 256, 16, 128, true, true, 16, 16),
+    GFFTX_TILE(PREC_F32, float, 256, 8, 128, true, false, 16, 16),
+    GFFTX_TILE(PREC_F32, float, 256, 8, 128, true, true, 16, 16),
+    GFFTX_TILE(PREC_F32, float, 256, 32, 256, true, false, 16, 16),
+    GFFTX_TILE(PREC_F32, float, 256, 32, 256, true, true, 16, 16),
+    GFFTX_TILE(PREC_F32, float, 256, 8, 64, true, false, 16, 16),
+    GFFTX_TILE(PREC_F32, float, 256, 8, 64, true, true, 16, 16),
+    GFFTX_TILE(PREC_F32, float, 4096, 4, 512, true, false, 16, 16, 16),
+    GFFTX_TILE(PREC_F32, float, 4096, 4, 512, true, true, 16, 16, 16),
+    GFFTX_TILE(PREC_F32, float, 4096, 4, 256, true, false, 16, 16, 16),
+    GFFTX_TILE(PREC_F32, float, 4096, 4, 256, true, true, 16, 16, 16),
     GFFTX_TILE(PREC_F32, float, 16, 64, 64, true, true, 16),
     GFFTX_TILE(PREC_F32, float, 64, 64, 256, true, true, 16, 4),
     GFFTX_TILE(PREC_F32, float, 16, 64, 64, true, false, 16),
@@ -122,6 +136,10 @@ TileEntry kTiles[] = {
     GFFTX_BATCH_POW2(PREC_F64, double),
     GFFTX_BATCH_MIXED(PREC_F64, double),
     GFFTX_PASS_SET(PREC_F64, double, 2, 512, 4, 256, 4, 384),
+    GFFTX_TILE(PREC_F64, double, 256, 8, 128, true, false, 16, 16),
+    GFFTX_TILE(PREC_F64, double, 256, 8, 128, true, true, 16, 16),
+    GFFTX_TILE(PREC_F64, double, 256, 16, 128, true, false, 16, 16),
+    GFFTX_TILE(PREC_F64, double, 256, 16, 128, true, true, 16, 16),
     GFFTX_TILE(PREC_F64, double, 4096, 1, 256, true, false, 16, 16, 16),
     GFFTX_TILE(PREC_F64, double, 4096, 1, 256, true, true, 16, 16, 16),
     GFFTX_TILE(PREC_F64, double, 16, 64, 64, true, true, 16),

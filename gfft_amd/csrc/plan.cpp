@@ -111,14 +111,15 @@ struct Range {
 
 size_t elem_bytes(int prec) { return prec == PREC_F64 ? 16 : 8; }
 
-// First registered tile for (prec, L, modes), or the one with nb == want_nb when present.
-const TileEntry* find_tile(int prec, int L, bool loadT, bool storeT, int want_nb = 0)
+// First registered tile for (prec, L, modes), or the first one with nb == want_nb and
+// wg == want_wg (each when nonzero) when there is one.
+const TileEntry* find_tile(int prec, int L, bool loadT, bool storeT, int want_nb = 0, int want_wg = 0)
 {
     const TileEntry* first = nullptr;
     for (int i = 0; i < kNumTiles; ++i) {
         const TileEntry& e = kTiles[i];
         if (e.prec == prec && e.L == L && e.loadT == loadT && e.storeT == storeT) {
-            if (want_nb && e.nb == want_nb) return &e;
+            if ((want_nb || want_wg) && (!want_nb || e.nb == want_nb) && (!want_wg || e.wg == want_wg)) return &e;
             if (!first) first = &e;
         }
     }
@@ -511,7 +512,7 @@ int build_c2c_passes(gfftx_plan_s* p)
         long long ns = 1;
         for (size_t i = 0; i < lens.size(); ++i) {
             const int L = lens[i];
-            const TileEntry* k = find_tile(p->prec, L, true, i > 0, env_int("GFFTX_TNB", 0));
+            const TileEntry* k = find_tile(p->prec, L, true, i > 0, env_int("GFFTX_TNB", 0), env_int("GFFTX_TWG", 0));
             if (!k) return fail(GFFTX_ERR_UNSUPPORTED, "no pass kernel L=%d", L);
             Pass ps;
             ps.k = k;
@@ -1175,6 +1176,10 @@ int create_plan_impl(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int
 
 // ---------------------------------------------------------------------------------------
 // Ragged batches: one persistent launch over a list of work items (ragged.hip).
+// Side streams for the long length classes: with the caller's stream they fill the four
+// hardware queues a process has by default.
+constexpr int kRaggedSides = 3;
+
 struct gfftx_ragged_s {
     int device = 0;
     bool inverse = false;
@@ -1182,7 +1187,23 @@ struct gfftx_ragged_s {
     unsigned long long total = 0;          // complex elements of the whole batch
     std::vector<void*> allocs;
     RaggedArgs args{};
-    int grid = 0;
+    int grid = 0, grid1 = 0;  // workgroups of the phase-0 and phase-1 launches
+    // Grouped mode (default): each length class above 4096 points runs as its own batched
+    // launch sequence (a single-pass tile through the sequence-offset table, or multi-pass
+    // tiles through a batch-offset table) on a side stream, while the persistent kernel
+    // takes the shorter lengths on the caller's stream.
+    struct LongClass {
+        long long L = 0;
+        long long count = 0;
+        const long long* offs = nullptr;  // element offsets of the class's transforms
+        std::vector<Pass> passes;         // one pass: single-pass tile over all of them
+    };
+    std::vector<LongClass> longs;
+    void* lscratch[2][kRaggedSides] = {};  // per side stream, the size of the whole batch: same offsets
+    int nsides = 0;                        // side streams in use (classes spread by work)
+    hipStream_t side[kRaggedSides] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[kRaggedSides] = {};
+    std::vector<int> side_of;              // side stream of each long class
     std::string desc;
     // hipGraph replay (GFFTX_FLAG_GRAPH), keyed by the data pointer
     GraphCache graphs;
@@ -1212,6 +1233,136 @@ int ragged_pass1_cfg(long long P)
     return -1;
 }
 
+// A length class above the persistent kernel's tiles: the transforms' offsets on the
+// device, and the pass list that computes all of them in one batched launch per pass.
+// A single-pass tile (one workgroup per transform, 16 bytes of HBM traffic per point)
+// is used when one exists and the class has at least `min_single` transforms; otherwise
+// multi-pass tiles (many workgroups per transform, 32 bytes per point).  The classes run
+// concurrently, so the lower traffic wins even for classes too small to fill the device
+// (measured on the 4096-transform batch).  `nscratch` is raised to the scratch arrays the
+// pass list needs.
+int ragged_long_class(gfftx_ragged_s* r, long long L, const std::vector<long long>& offs, long long min_single,
+                      int* nscratch)
+{
+    gfftx_ragged_s::LongClass lc;
+    lc.L = L;
+    lc.count = (long long)offs.size();
+    void* d = nullptr;
+    int e = dev_alloc_list(r->allocs, &d, offs.size() * sizeof(long long));
+    if (e) return e;
+    HIP_TRY(hipMemcpy(d, offs.data(), offs.size() * sizeof(long long), hipMemcpyHostToDevice));
+    lc.offs = (const long long*)d;
+    std::vector<int> lens;
+    const bool multi = choose_passes(PREC_F32, (unsigned long long)L, lens);
+    const TileEntry* k1 = (L <= (1 << 30)) ? find_tile(PREC_F32, (int)L, false, false) : nullptr;
+    if (k1 && (!multi || lc.count >= min_single)) {
+        const TileEntry* k = k1;
+        Pass ps;
+        ps.k = k;
+        ps.items_in_ntrans = true;
+        ps.ntrans = lc.count;
+        ps.ies = 1;
+        e = upload_stage_table_r(r->allocs, PREC_F32, k->nrad, k->radices, &ps.stage_tw);
+        if (e) return e;
+        lc.passes.push_back(ps);
+    } else {
+        if (!multi) return fail(GFFTX_ERR_UNSUPPORTED, "ragged batch: no kernel for length %lld", L);
+        long long ns = 1;
+        for (size_t i = 0; i < lens.size(); ++i) {
+            const TileEntry* k = find_tile(PREC_F32, lens[i], true, i > 0);
+            if (!k) return fail(GFFTX_ERR_UNSUPPORTED, "ragged batch: no pass kernel L=%d", lens[i]);
+            Pass ps;
+            ps.k = k;
+            ps.xcd = 1;
+            ps.ntrans = L / lens[i];
+            ps.its = 1;
+            ps.ies = L / lens[i];
+            ps.ns_pass = ns;
+            ps.batch = (unsigned)lc.count;
+            e = upload_stage_table_r(r->allocs, PREC_F32, k->nrad, k->radices, &ps.stage_tw);
+            if (e) return e;
+            if (ns > 1) {
+                ps.ptw_m = ns * lens[i];
+                e = upload_root_tables(r->allocs, ps.ptw_m, &ps.ptw_lo, &ps.ptw_hi, &ps.ptw_shift);
+                if (e) return e;
+            }
+            lc.passes.push_back(ps);
+            ns *= lens[i];
+        }
+        *nscratch = std::max(*nscratch, lens.size() > 2 ? 2 : 1);  // S0, and S1 from three passes
+    }
+    r->longs.push_back(std::move(lc));
+    return GFFTX_SUCCESS;
+}
+
+// Launches of one long class on stream s, in place on `data`.  Pass i reads the output of
+// pass i-1 and writes scratch S[i % 2], the last pass writes `data` (data -> S0 -> data,
+// data -> S0 -> S1 -> data, ...); the scratch arrays use the data offsets.
+int ragged_run_long(gfftx_ragged_s* r, const gfftx_ragged_s::LongClass& lc, float2* data, int side, hipStream_t s)
+{
+    const size_t m = lc.passes.size();
+    const void* src = data;
+    for (size_t i = 0; i < m; ++i) {
+        const Pass& ps = lc.passes[i];
+        void* dst = (i + 1 == m) ? (void*)data : r->lscratch[i % 2][side];
+        TileArgs a;
+        memset(&a, 0, sizeof(a));
+        a.in = src;
+        a.out = dst;
+        src = dst;
+        a.tw = ps.stage_tw;
+        a.conj_in = (r->inverse && i == 0) ? 1 : 0;
+        a.conj_out = (r->inverse && i == m - 1) ? 1 : 0;
+        a.scale = (r->inverse && i == m - 1) ? 1.0 / (double)lc.L : 1.0;
+        dim3 grid;
+        if (ps.items_in_ntrans) {
+            a.ntrans = lc.count;
+            a.ies = 1;
+            a.ns_pass = 1;
+            a.seq_off = lc.offs;
+            grid = dim3((unsigned)((lc.count + ps.k->nb - 1) / ps.k->nb));
+        } else {
+            a.ntrans = ps.ntrans;
+            a.its = ps.its;
+            a.ies = ps.ies;
+            a.ns_pass = ps.ns_pass;
+            a.boff = lc.offs;
+            a.ptw_lo = ps.ptw_lo;
+            a.ptw_hi = ps.ptw_hi;
+            a.ptw_shift = ps.ptw_shift;
+            a.xcd = ps.xcd;
+            grid = dim3((unsigned)((ps.ntrans + ps.k->nb - 1) / ps.k->nb), (unsigned)lc.count);
+        }
+        hipError_t le = ps.k->launch(a, grid, s);
+        if (le != hipSuccess) return fail(GFFTX_ERR_LAUNCH, "%s: %s", ps.k->name, hipGetErrorString(le));
+    }
+    return GFFTX_SUCCESS;
+}
+
+// Every launch of a ragged batch on stream s: the long classes on the side streams (forked
+// from and joined back into s; each side stream has its own scratch), the persistent
+// kernel on s.
+int ragged_launch_all(gfftx_ragged_s* r, const RaggedArgs& a, hipStream_t s)
+{
+    if (!r->longs.empty()) HIP_TRY(hipEventRecord(r->fork_ev, s));
+    if (a.nitems0 > 0) {
+        hipError_t le = launch_ragged(a, r->grid, r->grid1, s);
+        if (le != hipSuccess) return fail(GFFTX_ERR_LAUNCH, "ragged_kernel: %s", hipGetErrorString(le));
+    }
+    if (r->longs.empty()) return GFFTX_SUCCESS;
+    for (int k = 0; k < r->nsides; ++k) HIP_TRY(hipStreamWaitEvent(r->side[k], r->fork_ev, 0));
+    for (size_t i = 0; i < r->longs.size(); ++i) {
+        const int k = r->side_of[i];
+        int rc = ragged_run_long(r, r->longs[i], a.data, k, r->side[k]);
+        if (rc) return rc;
+    }
+    for (int k = 0; k < r->nsides; ++k) {
+        HIP_TRY(hipEventRecord(r->join_ev[k], r->side[k]));
+        HIP_TRY(hipStreamWaitEvent(s, r->join_ev[k], 0));
+    }
+    return GFFTX_SUCCESS;
+}
+
 int ragged_build(gfftx_ragged_s* r, const uint64_t* lengths, uint64_t count)
 {
     // classes by length, in order of first appearance
@@ -1237,7 +1388,15 @@ int ragged_build(gfftx_ragged_s* r, const uint64_t* lengths, uint64_t count)
         RaggedItem it;
         double cost;
     };
-    std::vector<HostItem> items;
+    std::vector<HostItem> items, items1;  // phase 0, phase 1
+    // lengths above 4096: batched launches per class (default), or inside the persistent
+    // kernel, both passes of a transform by one workgroup (GFFTX_RAGGED_MODE=1) or one
+    // launch per pass (GFFTX_RAGGED_MODE=2)
+    const int mode = env_int("GFFTX_RAGGED_MODE", 0);
+    const bool split = mode == 2;
+    int nscratch = 0;
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, r->device));
     long long scratch = 0;
     char buf[160];
     r->desc = "{\"transforms\":" + std::to_string(count) + ",\"points\":" + std::to_string(off) + ",\"classes\":[";
@@ -1248,7 +1407,32 @@ int ragged_build(gfftx_ragged_s* r, const uint64_t* lengths, uint64_t count)
         rc.L = L;
         rc.count = (long long)members[c].size();
         const int single = ragged_single_cfg(L);
-        const int p1 = (L % 64 == 0) ? ragged_pass1_cfg(L / 64) : -1;
+        if (single < 0 && mode == 0) {
+            int e = ragged_long_class(r, L, members[c], env_int("GFFTX_RAGGED_MIN_SINGLE", 0), &nscratch);
+            if (e) return e;
+            snprintf(buf, sizeof(buf), "%s{\"L\":%lld,\"count\":%lld,\"batched\":[", c ? "," : "", L, rc.count);
+            r->desc += buf;
+            for (size_t i = 0; i < r->longs.back().passes.size(); ++i)
+                r->desc += std::string(i ? ",\"" : "\"") + r->longs.back().passes[i].k->name + "\"";
+            r->desc += "]}";
+            continue;
+        }
+        // two-pass split L = A * P: the one whose smaller tile row (NB of either pass)
+        // is widest, so that both passes move long contiguous rows
+        int p1 = -1, cfg0 = -1;
+        long long A = 0;
+        for (int a0 : {RAGGED_CFG_T256, RAGGED_CFG_T128, RAGGED_CFG_T64}) {
+            const RaggedCfgInfo ca = ragged_cfg_info(a0);
+            if (L % ca.L) continue;
+            const int cp = ragged_pass1_cfg(L / ca.L);
+            if (cp < 0) continue;
+            const int w = std::min(ca.nb, ragged_cfg_info(cp).nb);
+            if (p1 < 0 || w > std::min(ragged_cfg_info(cfg0).nb, ragged_cfg_info(p1).nb)) {
+                p1 = cp;
+                cfg0 = a0;
+                A = ca.L;
+            }
+        }
         const double unit = (double)L * std::log2((double)L);
         if (single >= 0) {
             const RaggedCfgInfo ci = ragged_cfg_info(single);
@@ -1266,7 +1450,7 @@ int ragged_build(gfftx_ragged_s* r, const uint64_t* lengths, uint64_t count)
                 HostItem h;
                 memset(&h.it, 0, sizeof(h.it));
                 h.it.cfg = single;
-                h.it.cfg1 = -1;
+                h.it.kind = RAGGED_SINGLE;
                 h.it.cls = (int)c;
                 h.it.tile = (int)t;
                 h.cost = unit * (double)std::min<long long>(ci.nb, rc.count - t * ci.nb);
@@ -1275,7 +1459,8 @@ int ragged_build(gfftx_ragged_s* r, const uint64_t* lengths, uint64_t count)
             snprintf(buf, sizeof(buf), "%s{\"L\":%lld,\"count\":%lld,\"cfg\":%d,\"nb\":%d}", c ? "," : "", L,
                      rc.count, single, ci.nb);
         } else if (p1 >= 0) {
-            const RaggedCfgInfo c0 = ragged_cfg_info(RAGGED_CFG_T64), c1 = ragged_cfg_info(p1);
+            const RaggedCfgInfo c0 = ragged_cfg_info(cfg0), c1 = ragged_cfg_info(p1);
+            rc.A = (int)A;
             void* tw = nullptr;
             int e = upload_stage_table_r(r->allocs, PREC_F32, c0.nrad, c0.radices, &tw);
             if (e) return e;
@@ -1288,35 +1473,64 @@ int ragged_build(gfftx_ragged_s* r, const uint64_t* lengths, uint64_t count)
             if (e) return e;
             rc.ptw_lo = lo;
             rc.ptw_hi = hi;
+            const long long P = L / A;
+            const int nt0 = (int)((P + c0.nb - 1) / c0.nb), nt1 = (int)((A + c1.nb - 1) / c1.nb);
             for (long long o : members[c]) {
-                HostItem h;
-                memset(&h.it, 0, sizeof(h.it));
-                h.it.cfg = RAGGED_CFG_T64;
-                h.it.cfg1 = p1;
-                h.it.cls = (int)c;
-                h.it.ntiles0 = (int)((L / 64 + c0.nb - 1) / c0.nb);
-                h.it.ntiles1 = (64 + c1.nb - 1) / c1.nb;
-                h.it.off = o;
-                h.it.soff = scratch;
+                if (!split) {
+                    HostItem h;
+                    memset(&h.it, 0, sizeof(h.it));
+                    h.it.cfg = cfg0;
+                    h.it.cfg1 = p1;
+                    h.it.kind = RAGGED_FUSED;
+                    h.it.cls = (int)c;
+                    h.it.nt0 = nt0;
+                    h.it.nt1 = nt1;
+                    h.it.off = o;
+                    h.it.soff = scratch;
+                    h.cost = unit * 1.3;  // two passes, the second through L2
+                    items.push_back(h);
+                } else {
+                    for (int pass = 0; pass < 2; ++pass) {
+                        const int nt = pass ? nt1 : nt0;
+                        for (int t = 0; t < nt; ++t) {
+                            HostItem h;
+                            memset(&h.it, 0, sizeof(h.it));
+                            h.it.cfg = pass ? p1 : cfg0;
+                            h.it.kind = pass ? RAGGED_PASS1 : RAGGED_PASS0;
+                            h.it.cls = (int)c;
+                            h.it.tile = t;
+                            h.it.off = o;
+                            h.it.soff = scratch;
+                            // points of the tile times log2 of the FFT length it runs
+                            h.cost = pass ? (double)std::min<long long>(c1.nb, A - (long long)t * c1.nb) * P *
+                                                std::log2((double)P)
+                                          : (double)std::min<long long>(c0.nb, P - (long long)t * c0.nb) * A *
+                                                std::log2((double)A);
+                            (pass ? items1 : items).push_back(h);
+                        }
+                    }
+                }
                 scratch += L;
-                h.cost = unit * 1.3;  // two passes through L2 on one CU
-                items.push_back(h);
             }
-            snprintf(buf, sizeof(buf), "%s{\"L\":%lld,\"count\":%lld,\"cfg\":[%d,%d]}", c ? "," : "", L, rc.count,
-                     RAGGED_CFG_T64, p1);
+            snprintf(buf, sizeof(buf), "%s{\"L\":%lld,\"count\":%lld,\"split\":[%lld,%lld],\"cfg\":[%d,%d]}", c ? "," : "",
+                     L, rc.count, A, P, cfg0, p1);
         } else {
             return fail(GFFTX_ERR_UNSUPPORTED, "ragged batch: no kernel for length %lld", L);
         }
         r->desc += buf;
     }
-    // longest processing time first
-    std::stable_sort(items.begin(), items.end(), [](const HostItem& x, const HostItem& y) { return x.cost > y.cost; });
-    std::vector<RaggedItem> host_items(items.size());
-    for (size_t i = 0; i < items.size(); ++i) host_items[i] = items[i].it;
+    // longest processing time first, within each phase
+    auto by_cost = [](const HostItem& x, const HostItem& y) { return x.cost > y.cost; };
+    std::stable_sort(items.begin(), items.end(), by_cost);
+    std::stable_sort(items1.begin(), items1.end(), by_cost);
+    std::vector<RaggedItem> host_items;
+    host_items.reserve(items.size() + items1.size());
+    for (const HostItem& h : items) host_items.push_back(h.it);
+    for (const HostItem& h : items1) host_items.push_back(h.it);
     void *d_items = nullptr, *d_classes = nullptr, *d_counter = nullptr, *d_scratch = nullptr;
     int e = dev_alloc_list(r->allocs, &d_items, host_items.size() * sizeof(RaggedItem));
     if (!e) e = dev_alloc_list(r->allocs, &d_classes, classes.size() * sizeof(RaggedClass));
-    if (!e) e = dev_alloc_list(r->allocs, &d_counter, sizeof(unsigned long long));
+    if (!e) e = dev_alloc_list(r->allocs, &d_counter, 2 * sizeof(unsigned long long));
     if (!e) e = dev_alloc_list(r->allocs, &d_scratch, (size_t)std::max<long long>(scratch, 1) * sizeof(float2));
     if (e) return e;
     HIP_TRY(hipMemcpy(d_items, host_items.data(), host_items.size() * sizeof(RaggedItem), hipMemcpyHostToDevice));
@@ -1325,14 +1539,41 @@ int ragged_build(gfftx_ragged_s* r, const uint64_t* lengths, uint64_t count)
     r->args.classes = (const RaggedClass*)d_classes;
     r->args.counter = (unsigned long long*)d_counter;
     r->args.scratch = (float2*)d_scratch;
-    r->args.nitems = (int)host_items.size();
+    if (!r->longs.empty()) {
+        // classes to side streams, largest work first onto the least loaded stream
+        r->nsides = std::max(1, std::min<int>(env_int("GFFTX_RAGGED_STREAMS", kRaggedSides), kRaggedSides));
+        r->nsides = std::min<int>(r->nsides, (int)r->longs.size());
+        std::vector<size_t> order(r->longs.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+        auto work = [&](size_t i) { return (double)r->longs[i].L * (double)r->longs[i].count * std::log2((double)r->longs[i].L); };
+        std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return work(x) > work(y); });
+        std::vector<double> load(r->nsides, 0.0);
+        r->side_of.assign(r->longs.size(), 0);
+        for (size_t i : order) {
+            const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            r->side_of[i] = k;
+            load[k] += work(i);
+        }
+        for (int k = 0; k < r->nsides; ++k) {
+            HIP_TRY(hipStreamCreateWithFlags(&r->side[k], hipStreamNonBlocking));
+            HIP_TRY(hipEventCreateWithFlags(&r->join_ev[k], hipEventDisableTiming));
+            for (int b = 0; b < nscratch; ++b) {
+                e = dev_alloc_list(r->allocs, &r->lscratch[b][k], (size_t)off * sizeof(float2));
+                if (e) return e;
+            }
+        }
+        HIP_TRY(hipEventCreateWithFlags(&r->fork_ev, hipEventDisableTiming));
+    }
+    r->args.nitems0 = (int)items.size();
+    r->args.nitems1 = (int)items1.size();
     r->args.inverse = r->inverse ? 1 : 0;
-    hipDeviceProp_t prop;
-    HIP_TRY(hipGetDeviceProperties(&prop, r->device));
     const int per_cu = env_int("GFFTX_RAGGED_WG_PER_CU", 3);
-    r->grid = std::max(1, std::min<int>((int)host_items.size(), prop.multiProcessorCount * per_cu));
-    r->desc += "],\"items\":" + std::to_string(host_items.size()) + ",\"grid\":" + std::to_string(r->grid) +
-               ",\"scratch_points\":" + std::to_string(scratch) + "}";
+    const int cus = prop.multiProcessorCount;
+    r->grid = std::max(1, std::min<int>((int)items.size(), cus * per_cu));
+    r->grid1 = std::max(1, std::min<int>((int)items1.size(), cus * per_cu));
+    r->desc += "],\"items\":[" + std::to_string(items.size()) + "," + std::to_string(items1.size()) +
+               "],\"grid\":[" + std::to_string(r->grid) + "," + std::to_string(r->grid1) +
+               "],\"scratch_points\":" + std::to_string(scratch) + "}";
     return GFFTX_SUCCESS;
 }
 
@@ -1358,12 +1599,12 @@ int ragged_launch(gfftx_ragged_s* r, void* data, hipStream_t s)
         if (!ge) {
             if (!r->graphs.cap_stream) HIP_TRY(hipStreamCreateWithFlags(&r->graphs.cap_stream, hipStreamNonBlocking));
             HIP_TRY(hipStreamBeginCapture(r->graphs.cap_stream, hipStreamCaptureModeThreadLocal));
-            hipError_t le = launch_ragged(a, r->grid, r->graphs.cap_stream);
+            const int lrc = ragged_launch_all(r, a, r->graphs.cap_stream);
             hipGraph_t g = nullptr;
             hipError_t ce = hipStreamEndCapture(r->graphs.cap_stream, &g);
-            if (le != hipSuccess) {
+            if (lrc) {
                 if (g) (void)hipGraphDestroy(g);
-                return fail(GFFTX_ERR_LAUNCH, "ragged capture: %s", hipGetErrorString(le));
+                return lrc;
             }
             if (ce != hipSuccess) return fail(GFFTX_ERR_DEVICE, "end capture: %s", hipGetErrorString(ce));
             ge = r->graphs.take();
@@ -1379,8 +1620,8 @@ int ragged_launch(gfftx_ragged_s* r, void* data, hipStream_t s)
         }
         HIP_TRY(hipGraphLaunch(ge->exec, s));
     } else {
-        hipError_t le = launch_ragged(a, r->grid, s);
-        if (le != hipSuccess) return fail(GFFTX_ERR_LAUNCH, "ragged_kernel: %s", hipGetErrorString(le));
+        const int lrc = ragged_launch_all(r, a, s);
+        if (lrc) return lrc;
     }
     if (prof) {
         HIP_TRY(hipEventRecord(ev.second, s));
@@ -1717,8 +1958,7 @@ int gfftx_ragged_create(gfftx_ragged* plan, const uint64_t* lengths, uint64_t co
     r->flags = flags;
     int rc = ragged_build(r, lengths, count);
     if (rc != GFFTX_SUCCESS) {
-        for (void* a : r->allocs) (void)hipFree(a);
-        delete r;
+        gfftx_ragged_destroy(r);  // leaves the error message alone
         return rc;
     }
     *plan = r;
@@ -1777,6 +2017,11 @@ int gfftx_ragged_destroy(gfftx_ragged plan)
     if (!plan) return GFFTX_SUCCESS;
     DeviceGuard g(plan->device);
     plan->graphs.clear();
+    for (int k = 0; k < kRaggedSides; ++k) {
+        if (plan->side[k]) (void)hipStreamDestroy(plan->side[k]);
+        if (plan->join_ev[k]) (void)hipEventDestroy(plan->join_ev[k]);
+    }
+    if (plan->fork_ev) (void)hipEventDestroy(plan->fork_ev);
     for (auto& ev : plan->pending) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
     for (auto& ev : plan->pool) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
     for (void* a : plan->allocs) (void)hipFree(a);

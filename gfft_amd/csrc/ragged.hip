@@ -1,16 +1,18 @@
-// ragged.hip -- many transforms of different lengths in one persistent launch (gfx950).
+// ragged.hip -- many transforms of different lengths in one or two persistent launches.
 //
-// The host turns a ragged batch into a list of work items, longest first:
-//   * lengths up to 4096: one item is one tile of NB transforms of the same length; the
-//     transforms stay where the caller put them and are reached through an offset table;
-//   * longer lengths (L = 64*P, P <= 1024): one item is one whole transform, computed by
-//     one workgroup in two passes of the four-step method: 64-point FFTs of the P
-//     sequences x[j + P r] (64 adjacent sequences per tile, so every row access is 512
-//     bytes) stored transposed into a scratch region, then P-point FFTs of its contiguous
-//     rows with the twiddle W_L^{j r}, stored to X[j + 64 q].  No workgroup ever waits for
-//     another.
-// Workgroups take items from a global counter until the list is exhausted, so every
-// workgroup ends after at most (items + 1) fetches.
+// The host turns a ragged batch into work lists sorted longest first:
+//   * lengths up to 4096: tiles of NB transforms of one length (the transforms stay where
+//     the caller put them and are reached through an offset table);
+//   * longer lengths L = A*P (A in {64, 128, 256}, chosen so that both passes move rows of
+//     128 bytes or more): the four-step method, A-point FFTs of the P sequences
+//     x[j + P r] stored transposed into the transform's scratch region, then P-point FFTs
+//     of the contiguous scratch rows with the twiddle W_L^{j r}, stored to X[j + A q].
+//     Either one workgroup runs both passes of a transform in turn (fused: the scratch
+//     region stays in L2), or the first-pass tiles of every transform run in the first
+//     launch and the second-pass tiles in a second one (split).
+// Workgroups take items from their launch's counter until the list is exhausted, so every
+// workgroup ends after at most (items + 1) fetches; the launch boundary orders the two
+// passes of a split transform, and no workgroup ever waits for another.
 #include "ragged.h"
 
 namespace gfftx {
@@ -53,7 +55,9 @@ namespace gfftx {
     X(31, RCFG(640, 6, 240, false, true, 16, 8, 5))    \
     X(32, RCFG(768, 5, 240, false, true, 16, 16, 3))   \
     X(33, RCFG(1024, 4, 256, false, true, 16, 16, 4))  \
-    X(34, RCFG(64, 64, 256, true, true, 16, 4))
+    X(34, RCFG(64, 64, 256, true, true, 16, 4))        \
+    X(35, RCFG(128, 32, 256, true, true, 16, 8))       \
+    X(36, RCFG(256, 16, 256, true, true, 16, 16))
 
 constexpr int ragged_lds_elems()
 {
@@ -83,85 +87,109 @@ __device__ __forceinline__ void run_cfg(int cfg, const TileArgs& a, unsigned til
     }
 }
 
-__global__ __launch_bounds__(RAGGED_WG) void ragged_reset_kernel(unsigned long long* counter) { *counter = 0; }
+__global__ __launch_bounds__(RAGGED_WG) void ragged_reset_kernel(unsigned long long* counter)
+{
+    counter[0] = 0;
+    counter[1] = 0;
+}
 
 __global__ __launch_bounds__(RAGGED_WG) void ragged_kernel(RaggedArgs ra)
 {
     __shared__ float2 lds[kRaggedLds];
     __shared__ int cur;
+    const RaggedItem* list = ra.phase ? ra.items + ra.nitems0 : ra.items;
+    const int nitems = ra.phase ? ra.nitems1 : ra.nitems0;
+    unsigned long long* counter = ra.counter + ra.phase;
     for (;;) {
-        if (threadIdx.x == 0) cur = (int)atomicAdd(ra.counter, 1ull);
+        if (threadIdx.x == 0) cur = (int)atomicAdd(counter, 1ull);
         __syncthreads();
         const int it = cur;
         __syncthreads();
-        if (it >= ra.nitems) break;
-        const RaggedItem item = ra.items[it];
+        if (it >= nitems) break;
+        const RaggedItem item = list[it];
         const RaggedClass cls = ra.classes[item.cls];
         TileArgs a;
         a.its = 0;
         a.bdist = 0;
         a.xcd = 0;
-        a.conj_in = ra.inverse;
+        a.conj_in = 0;
         a.conj_out = 0;
         a.ptw_lo = nullptr;
         a.ptw_hi = nullptr;
         a.ptw_shift = 0;
-        a.tw = cls.tw0;
-        a.ies = 1;
-        a.ns_pass = 1;
-        if (item.cfg1 < 0) {
+        a.seq_off = nullptr;
+        a.boff = nullptr;
+        a.scale = 1.0;
+        if (item.kind == RAGGED_SINGLE) {
             // tile of same-length transforms, in place through the offset table
             a.in = ra.data;
             a.out = ra.data;
             a.ntrans = cls.count;
+            a.ies = 1;
+            a.ns_pass = 1;
+            a.tw = cls.tw0;
             a.seq_off = cls.seq_off;
+            a.conj_in = ra.inverse;
             a.conj_out = ra.inverse;
             a.scale = ra.inverse ? 1.0 / (double)cls.L : 1.0;
-            run_cfg(item.cfg, a, (unsigned)item.tile, lds);
         } else {
-            // pass 0: 64-point FFTs of the P sequences x[j + P r] -> scratch z[q P + j]
-            const long long P = cls.L / 64;
+            // first pass of a two-pass transform: A-point FFTs of x[j + P r] -> z[q P + j]
+            const long long A = cls.A, P = cls.L / A;
             a.in = ra.data + item.off;
             a.out = ra.scratch + item.soff;
             a.ntrans = P;
             a.its = 1;
             a.ies = P;
             a.ns_pass = P;
-            a.seq_off = nullptr;
-            a.scale = 1.0;
-            for (int t = 0; t < item.ntiles0; ++t) {
-                run_cfg(item.cfg, a, (unsigned)t, lds);
-                __syncthreads();
+            a.tw = cls.tw0;
+            a.conj_in = ra.inverse;
+            if (item.kind == RAGGED_FUSED) {
+                for (int t = 0; t < item.nt0; ++t) {
+                    run_cfg(item.cfg, a, (unsigned)t, lds);
+                    __syncthreads();
+                }
             }
-            // pass 1: P-point FFTs of the rows z[j P + r] times W_L^{j r} -> X[j + 64 q]
-            a.in = ra.scratch + item.soff;
-            a.out = ra.data + item.off;
-            a.ntrans = 64;
-            a.its = P;
-            a.ies = 1;
-            a.ns_pass = 64;
-            a.tw = cls.tw1;
-            a.ptw_lo = cls.ptw_lo;
-            a.ptw_hi = cls.ptw_hi;
-            a.ptw_shift = cls.ptw_shift;
-            a.conj_in = 0;
-            a.conj_out = ra.inverse;
-            a.scale = ra.inverse ? 1.0 / (double)cls.L : 1.0;
-            for (int t = 0; t < item.ntiles1; ++t) {
-                run_cfg(item.cfg1, a, (unsigned)t, lds);
-                __syncthreads();
+            if (item.kind != RAGGED_PASS0) {
+                // second pass: P-point FFTs of the rows z[j P + r] times W_L^{j r} -> X[j + A q]
+                a.in = ra.scratch + item.soff;
+                a.out = ra.data + item.off;
+                a.ntrans = A;
+                a.its = P;
+                a.ies = 1;
+                a.ns_pass = A;
+                a.tw = cls.tw1;
+                a.ptw_lo = cls.ptw_lo;
+                a.ptw_hi = cls.ptw_hi;
+                a.ptw_shift = cls.ptw_shift;
+                a.conj_in = 0;
+                a.conj_out = ra.inverse;
+                a.scale = ra.inverse ? 1.0 / (double)cls.L : 1.0;
+            }
+            if (item.kind == RAGGED_FUSED) {
+                for (int t = 0; t < item.nt1; ++t) {
+                    run_cfg(item.cfg1, a, (unsigned)t, lds);
+                    __syncthreads();
+                }
+                continue;
             }
         }
+        run_cfg(item.cfg, a, (unsigned)item.tile, lds);
         __syncthreads();
     }
 }
 
-hipError_t launch_ragged(const RaggedArgs& ra, int grid, hipStream_t s)
+hipError_t launch_ragged(const RaggedArgs& ra, int grid0, int grid1, hipStream_t s)
 {
     hipLaunchKernelGGL(ragged_reset_kernel, dim3(1), dim3(1), 0, s, ra.counter);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(ragged_kernel, dim3(grid), dim3(RAGGED_WG), 0, s, ra);
+    RaggedArgs a = ra;
+    a.phase = 0;
+    hipLaunchKernelGGL(ragged_kernel, dim3(grid0), dim3(RAGGED_WG), 0, s, a);
+    e = hipGetLastError();
+    if (e != hipSuccess || ra.nitems1 == 0) return e;
+    a.phase = 1;
+    hipLaunchKernelGGL(ragged_kernel, dim3(grid1), dim3(RAGGED_WG), 0, s, a);
     return hipGetLastError();
 }
 

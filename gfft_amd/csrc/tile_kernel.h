@@ -42,7 +42,13 @@ struct TileArgs {
     int xcd;            // 1: remap workgroups so neighbouring tiles share one XCD's L2
     double scale;
     const long long* seq_off;  // optional: element offset of every sequence (E-mode in/out)
+    const long long* boff;     // optional: element offset of every batch item (else by*bdist)
 };
+
+__device__ __forceinline__ long long batch_offset(const TileArgs& a, unsigned by)
+{
+    return a.boff ? a.boff[by] : (long long)by * a.bdist;
+}
 
 // Workgroups are dispatched round-robin over the 8 XCDs, so consecutive blockIdx values land
 // on different L2 caches.  With xcd remapping, tile (b%8)*(full/8) + b/8 is processed by
@@ -284,8 +290,9 @@ __device__ __forceinline__ Ctx<C> make_ctx(const TileArgs& a, unsigned tile, uns
     x.tg = (long long)tile * C::NB + x.t;
     x.valid = x.tg < a.ntrans;
     const long long tl = x.valid ? x.tg : a.ntrans - 1;
-    const V* in = reinterpret_cast<const V*>(a.in) + (long long)by * a.bdist;
-    V* out = reinterpret_cast<V*>(a.out) + (long long)by * a.bdist;
+    const long long bo = batch_offset(a, by);
+    const V* in = reinterpret_cast<const V*>(a.in) + bo;
+    V* out = reinterpret_cast<V*>(a.out) + bo;
     x.in = in + (a.seq_off ? a.seq_off[tl] : tl * a.its);
     x.out = out + out_base<C>(a, tl);
     x.ies = (unsigned)a.ies;
@@ -307,7 +314,7 @@ template <class C, bool IDLE> __device__ __forceinline__ void tile_finish(const 
         const auto sc = (decltype(V{}.x))a.scale;
         const bool scale = a.scale != 1.0;
         const int tid = threadIdx.x;
-        V* out = reinterpret_cast<V*>(a.out) + (long long)by * a.bdist;
+        V* out = reinterpret_cast<V*>(a.out) + batch_offset(a, by);
         if (!IDLE || x.active) {
 #pragma unroll
             for (int i = 0; i < PER; ++i) {

@@ -1,6 +1,9 @@
-"""GPU parity of the ragged batch (many lengths, one persistent launch) through the C ABI.
+"""GPU parity of the ragged batch (many lengths in one call) through the C ABI.
 
-Every transform is compared with the long-double oracle at relL2 <= 2e-7 (FP32)."""
+Every transform is compared with the long-double oracle at relL2 <= 2e-7 (FP32).  Lengths
+above 4096 run as batched launches per length class (GFFTX_RAGGED_MODE=0, the default) or
+inside the persistent kernel, fused (1) or split into two launches (2); all three modes
+are checked."""
 import numpy as np
 import pytest
 
@@ -37,7 +40,12 @@ def check_all(y, x, lengths, inverse=False, tol=2e-7):
     return worst
 
 
-def test_all_lengths_once(torch_cuda):
+MODES = ["0", "1", "2"]
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_all_lengths_once(torch_cuda, monkeypatch, mode):
+    monkeypatch.setenv("GFFTX_RAGGED_MODE", mode)
     lengths = [16, 32, 48, 64, 80, 96, 128, 160, 192] + C4_LENGTHS + C4_LENGTHS[::-1]
     assert len(C4_LENGTHS) == 25
     x = inputs.complex_normal(sum(lengths), 77).astype(np.complex64)
@@ -49,7 +57,9 @@ def test_all_lengths_once(torch_cuda):
     p.close()
 
 
-def test_inverse_round_trip(torch_cuda):
+@pytest.mark.parametrize("mode", MODES)
+def test_inverse_round_trip(torch_cuda, monkeypatch, mode):
+    monkeypatch.setenv("GFFTX_RAGGED_MODE", mode)
     lengths = [256, 5120, 65536, 384, 640, 12288]
     x = inputs.complex_normal(sum(lengths), 78).astype(np.complex64)
     pf = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT)
@@ -74,7 +84,9 @@ def test_unsupported_lengths(torch_cuda):
         api.RaggedPlan([256], api.COMPLEX_DOUBLE)
 
 
-def test_graph_replays_match_eager(torch_cuda):
+@pytest.mark.parametrize("mode", ["0", "1"])
+def test_graph_replays_match_eager(torch_cuda, monkeypatch, mode):
+    monkeypatch.setenv("GFFTX_RAGGED_MODE", mode)
     lengths = list(inputs.ragged_lengths(300, 11))
     x = inputs.complex_normal(int(sum(lengths)), 12).astype(np.complex64)
     eager = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT)
