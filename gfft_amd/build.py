@@ -16,6 +16,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgfftx.so")
 
 SOURCES = ["kernels.hip", "ragged.hip", "bluestein.hip", "plan.cpp"]
+# per-source flags: the MFMA kernels keep their products in VGPRs (no accumulator copies)
+EXTRA = {"kernels.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950",
@@ -36,7 +38,7 @@ def build(verbose: bool = False) -> str:
     for src in SOURCES:  # independent translation units compile in parallel
         obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
-        cmd = [HIPCC, *FLAGS, *lang, "-I", os.path.join(ROOT, "include"), "-c",
+        cmd = [HIPCC, *FLAGS, *EXTRA.get(src, []), *lang, "-I", os.path.join(ROOT, "include"), "-c",
                os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)

@@ -115,18 +115,8 @@ TileEntry kTiles[] = {
     GFFTX_TILE(PREC_F32, float, 4096, 1, 256, true, true, 16, 16, 16),
     GFFTX_TILE(PREC_F32, float, 256, 32, 512, true, false, 16, 16),
     GFFTX_TILE(PREC_F32, float, 256, 32, 512, true, true, 16, 16),
-    GFFTX_TILE(PREC_F32, float, 256, 16, 128, true, false, 16, 16),
-    GFFTX_TILE(PREC_F32, float, 256, 16, 128, true, true, 16, 16),
-    GFFTX_TILE(PREC_F32, float, 256, 8, 128, true, false, 16, 16),
-    GFFTX_TILE(PREC_F32, float, 256, 8, 128, true, true, 16, 16),
     GFFTX_TILE(PREC_F32, float, 256, 32, 256, true, false, 16, 16),
     GFFTX_TILE(PREC_F32, float, 256, 32, 256, true, true, 16, 16),
-    GFFTX_TILE(PREC_F32, float, 256, 8, 64, true, false, 16, 16),
-    GFFTX_TILE(PREC_F32, float, 256, 8, 64, true, true, 16, 16),
-    GFFTX_TILE(PREC_F32, float, 4096, 4, 512, true, false, 16, 16, 16),
-    GFFTX_TILE(PREC_F32, float, 4096, 4, 512, true, true, 16, 16, 16),
-    GFFTX_TILE(PREC_F32, float, 4096, 4, 256, true, false, 16, 16, 16),
-    GFFTX_TILE(PREC_F32, float, 4096, 4, 256, true, true, 16, 16, 16),
     GFFTX_TILE(PREC_F32, float, 16, 64, 64, true, true, 16),
     GFFTX_TILE(PREC_F32, float, 64, 64, 256, true, true, 16, 4),
     GFFTX_TILE(PREC_F32, float, 16, 64, 64, true, false, 16),
@@ -138,8 +128,6 @@ TileEntry kTiles[] = {
     GFFTX_PASS_SET(PREC_F64, double, 2, 512, 4, 256, 4, 384),
     GFFTX_TILE(PREC_F64, double, 256, 8, 128, true, false, 16, 16),
     GFFTX_TILE(PREC_F64, double, 256, 8, 128, true, true, 16, 16),
-    GFFTX_TILE(PREC_F64, double, 256, 16, 128, true, false, 16, 16),
-    GFFTX_TILE(PREC_F64, double, 256, 16, 128, true, true, 16, 16),
     GFFTX_TILE(PREC_F64, double, 4096, 1, 256, true, false, 16, 16, 16),
     GFFTX_TILE(PREC_F64, double, 4096, 1, 256, true, true, 16, 16, 16),
     GFFTX_TILE(PREC_F64, double, 16, 64, 64, true, true, 16),

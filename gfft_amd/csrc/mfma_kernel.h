@@ -22,6 +22,8 @@
 namespace gfftx {
 
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+typedef float float2_t __attribute__((ext_vector_type(2)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 // A operands in device memory: [mt][part][lane][8] with mt = 0 (Re rows) / 1 (Im rows) and
@@ -67,10 +69,16 @@ struct MfmaArgs {
 
 __device__ __forceinline__ float col_max4(float m)
 {
-    // lanes c, c+16, c+32, c+48 hold the same column
-    m = fmaxf(m, __shfl_xor(m, 16, 64));
-    m = fmaxf(m, __shfl_xor(m, 32, 64));
-    return m;
+    // Lanes c, c+16, c+32, c+48 hold the same column.  m >= 0, so the order of its bit
+    // patterns as unsigned integers is the order of the values; the two lane swaps
+    // (gfx950 v_permlane32_swap / v_permlane16_swap) exchange halves of the wave on the
+    // VALU, without a round trip through LDS.
+    unsigned u = __float_as_uint(m);
+    const auto a = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+    u = max(a[0], a[1]);  // max(m[l], m[l ^ 32])
+    const auto b = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+    u = max(b[0], b[1]);  // and of m[l ^ 16], m[l ^ 48]
+    return __uint_as_float(u);
 }
 
 template <class C, int S>
@@ -106,24 +114,27 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
             for (int i = 0; i < 4; ++i)
                 if (g != 0 || i != 0) v[i] = cmul(v[i], row[i]);
         }
-        // per-column power-of-two scale: largest entry just below 2^14
+        // per-column power-of-two scale: largest entry just below 2^14 (the exponent is
+        // clamped so that the scale and its inverse are normal FP32 numbers)
         float m = 0.f;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) m = fmaxf(m, fmaxf(fabsf(v[i].x), fabsf(v[i].y)));
+        for (int i = 0; i < 4; ++i) m = fmaxf(fmaxf(m, fabsf(v[i].x)), fabsf(v[i].y));  // one v_max3 per point
         m = col_max4(m);
         int ex = 0;
         (void)frexpf(m, &ex);
-        const int sh = 14 - ex;
+        const int sh = min(max(14 - ex, -126), 126);
+        const float sc = ldexpf(1.0f, sh), inv = ldexpf(1.0f, -sh);
         half8_t bh, bl;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const float xr = ldexpf(v[i].x, sh), xi = ldexpf(v[i].y, sh);
-            const _Float16 hr = (_Float16)xr, hi = (_Float16)xi;
-            bh[2 * i] = hr;
-            bh[2 * i + 1] = hi;
-            // x - hi is exact in FP32; one mixed-precision FMA computes it from the FP16 value
-            bl[2 * i] = (_Float16)__builtin_fmaf((float)hr, -1.0f, xr);
-            bl[2 * i + 1] = (_Float16)__builtin_fmaf((float)hi, -1.0f, xi);
+            // hi = FP16(x s); lo = FP16(x s - hi): x s - hi is formed by one mixed-precision
+            // FMA from x, s and the FP16 hi (the product by a power of two is exact)
+            const float2_t xs = {v[i].x * sc, v[i].y * sc};
+            const half2_t h = __builtin_convertvector(xs, half2_t);
+            bh[2 * i] = h.x;
+            bh[2 * i + 1] = h.y;
+            bl[2 * i] = (_Float16)__builtin_fmaf(v[i].x, sc, -(float)h.x);
+            bl[2 * i + 1] = (_Float16)__builtin_fmaf(v[i].y, sc, -(float)h.y);
         }
         float4_t acc[2];
 #pragma unroll
@@ -135,7 +146,7 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
             acc[mt] = z;
         }
 #pragma unroll
-        for (int i = 0; i < 4; ++i) y[b][i] = float2{ldexpf(acc[0][i], -sh), ldexpf(acc[1][i], -sh)};
+        for (int i = 0; i < 4; ++i) y[b][i] = float2{acc[0][i] * inv, acc[1][i] * inv};
     }
 
     if constexpr (!FIRST) __syncthreads();

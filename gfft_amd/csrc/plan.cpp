@@ -132,6 +132,19 @@ int env_int(const char* name, int dflt)
     return v ? atoi(v) : dflt;
 }
 
+// Compute units of a device (256 on MI355X when the query fails).
+int device_cus(int device)
+{
+    static int cached[64] = {0};
+    if (device >= 0 && device < 64 && cached[device]) return cached[device];
+    hipDeviceProp_t prop;
+    int cus = 256;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) cus = prop.multiProcessorCount;
+    else (void)hipGetLastError();
+    if (device >= 0 && device < 64) cached[device] = cus;
+    return cus;
+}
+
 bool only_235(unsigned long long n)
 {
     for (unsigned long long p : {2ull, 3ull, 5ull})
@@ -152,10 +165,25 @@ double pass_bandwidth(int prec, int L)
     return f32 ? 2.6 : 2.5;
 }
 
+// Fraction of the device one pass of length L keeps busy: its tiles (n/L sequences per
+// batch item, NB per tile) over the workgroups all CUs hold at once (LDS and threads
+// limit a CU to 160 KB and 2048 threads).
+double pass_occupancy(int prec, int L, bool first, unsigned long long n, unsigned long long batch, int cus)
+{
+    const TileEntry* k = find_tile(prec, L, true, !first);
+    if (!k) return 1.0;
+    const int by_lds = k->lds_bytes > 0 ? (160 * 1024) / k->lds_bytes : 8;
+    const int per_cu = std::max(1, std::min(by_lds, 2048 / k->wg));
+    const double tiles = std::ceil((double)(n / (unsigned long long)L) / k->nb) * (double)batch;
+    return std::min(1.0, tiles / ((double)cus * per_cu));
+}
+
 // Pass splits of n into compiled T-mode tile lengths (2..4 passes), best `k` by the cost
 // model: each pass moves 2 n b_c bytes at the bandwidth its tile reaches
-// (pass_bandwidth()).  The first length of every split has a T->E tile.
-std::vector<std::vector<int>> candidate_passes(int prec, unsigned long long n, size_t k)
+// (pass_bandwidth()) times the fraction of the device its tiles occupy.  The first length
+// of every split has a T->E tile.
+std::vector<std::vector<int>> candidate_passes(int prec, unsigned long long n, size_t k, unsigned long long batch = 1,
+                                               int cus = 256)
 {
     std::vector<int> lens;
     for (int i = 0; i < kNumTiles; ++i)
@@ -175,7 +203,8 @@ std::vector<std::vector<int>> candidate_passes(int prec, unsigned long long n, s
                 }
             if (!find_tile(prec, order[0], true, false)) return;
             double cost = 0;
-            for (int v : order) cost += 1.0 / pass_bandwidth(prec, v);
+            for (size_t i = 0; i < order.size(); ++i)
+                cost += 1.0 / (pass_bandwidth(prec, order[i]) * pass_occupancy(prec, order[i], i == 0, n, batch, cus));
             found.push_back({cost, order});
             return;
         }
@@ -197,9 +226,24 @@ std::vector<std::vector<int>> candidate_passes(int prec, unsigned long long n, s
     return out;
 }
 
+// A plan choice is the pass lengths, optionally followed by 0 and one (nb, wg) pair per
+// pass naming the tile to use ((0, 0): the default tile of that length and mode).
+void split_choice(const std::vector<int>& c, std::vector<int>& lens, std::vector<std::pair<int, int>>& tiles)
+{
+    lens.clear();
+    tiles.clear();
+    size_t i = 0;
+    while (i < c.size() && c[i] != 0) lens.push_back(c[i++]);
+    if (i < c.size()) ++i;
+    while (i + 1 < c.size()) {
+        tiles.push_back({c[i], c[i + 1]});
+        i += 2;
+    }
+}
+
 // Split n into pass lengths available as T-mode tiles (largest first).  An override list
 // (GFFTX_PASSES="a,b,c") is honoured when it multiplies to n.
-bool choose_passes(int prec, unsigned long long n, std::vector<int>& out)
+bool choose_passes(int prec, unsigned long long n, std::vector<int>& out, unsigned long long batch = 1, int cus = 256)
 {
     out.clear();
     if (const char* env = getenv("GFFTX_PASSES")) {
@@ -222,7 +266,7 @@ bool choose_passes(int prec, unsigned long long n, std::vector<int>& out)
             return true;
         }
     }
-    std::vector<std::vector<int>> c = candidate_passes(prec, n, 1);
+    std::vector<std::vector<int>> c = candidate_passes(prec, n, 1, batch, cus);
     if (c.empty()) return false;
     out = c[0];
     return true;
@@ -293,7 +337,7 @@ struct gfftx_plan_s {
     bool real = false;
     bool inverse = false;
     unsigned long long dist = 0;   // complex elements between batch items
-    std::vector<int> forced_lens;  // pass split chosen by the autotuner (empty: cost model)
+    std::vector<int> forced_lens;  // plan choice made by the autotuner (empty: cost model), see split_choice
     std::string tune_note;         // autotuner record for plan_describe
     std::vector<Pass> passes;
     std::vector<void*> allocs;     // device allocations owned by the plan
@@ -506,13 +550,17 @@ int build_c2c_passes(gfftx_plan_s* p)
         snprintf(buf, sizeof(buf), "{\"kernel\":\"%s\",\"L\":%d,\"ntrans\":%lld}", k->name, k->L, ps.ntrans);
         p->desc += buf;
     } else {
-        std::vector<int> lens = p->forced_lens;
-        if (lens.empty() && !choose_passes(p->prec, n, lens))
+        std::vector<int> lens;
+        std::vector<std::pair<int, int>> tiles;
+        split_choice(p->forced_lens, lens, tiles);
+        if (lens.empty() && !choose_passes(p->prec, n, lens, p->batch, device_cus(p->device)))
             return fail(GFFTX_ERR_UNSUPPORTED, "no multi-pass decomposition for n=%llu", n);
         long long ns = 1;
         for (size_t i = 0; i < lens.size(); ++i) {
             const int L = lens[i];
-            const TileEntry* k = find_tile(p->prec, L, true, i > 0, env_int("GFFTX_TNB", 0), env_int("GFFTX_TWG", 0));
+            const int want_nb = i < tiles.size() && tiles[i].first ? tiles[i].first : env_int("GFFTX_TNB", 0);
+            const int want_wg = i < tiles.size() && tiles[i].first ? tiles[i].second : env_int("GFFTX_TWG", 0);
+            const TileEntry* k = find_tile(p->prec, L, true, i > 0, want_nb, want_wg);
             if (!k) return fail(GFFTX_ERR_UNSUPPORTED, "no pass kernel L=%d", L);
             Pass ps;
             ps.k = k;
@@ -1005,7 +1053,7 @@ unsigned long long bluestein_max_m(int prec) { return prec == PREC_F64 ? (1ull <
 // Convolution length for the chirp-z path: the 2/3/5-smooth m >= 2n-1 (up to twice that)
 // with the lowest modelled cost, m times the sum over its passes of 1/bandwidth (a
 // single-pass tile counts as one pass at 4 TB/s).  0 when none can be planned.
-unsigned long long bluestein_length(int prec, unsigned long long n)
+unsigned long long bluestein_length(int prec, unsigned long long n, unsigned long long batch)
 {
     const unsigned long long lo = 2 * n - 1, hi = 2 * lo, maxm = bluestein_max_m(prec);
     std::vector<unsigned long long> cand;
@@ -1021,7 +1069,7 @@ unsigned long long bluestein_length(int prec, unsigned long long n)
         if (m <= (1ull << 30) && find_tile(prec, (int)m, false, false)) {
             per = 1.0 / 4.0;
         } else {
-            const std::vector<std::vector<int>> c = candidate_passes(prec, m, 1);
+            const std::vector<std::vector<int>> c = candidate_passes(prec, m, 1, batch);
             if (c.empty()) continue;
             for (int L : c[0]) per += 1.0 / pass_bandwidth(prec, L);
         }
@@ -1055,7 +1103,7 @@ int build_bluestein(gfftx_plan_s* p)
     if (n < 2 || 2 * n - 1 > bluestein_max_m(p->prec))
         return fail(GFFTX_ERR_UNSUPPORTED, "length %llu with a prime factor above 5: the chirp-z path takes 2 <= n <= %llu",
                     n, (bluestein_max_m(p->prec) + 1) / 2);
-    const unsigned long long m = bluestein_length(p->prec, n);
+    const unsigned long long m = bluestein_length(p->prec, n, p->batch);
     if (!m) return fail(GFFTX_ERR_UNSUPPORTED, "no convolution length for n=%llu", n);
     p->blue_m = m;
     const int cvt = p->prec == PREC_F64 ? GFFTX_COMPLEX_DOUBLE : GFFTX_COMPLEX_FLOAT;
@@ -1253,7 +1301,7 @@ int ragged_long_class(gfftx_ragged_s* r, long long L, const std::vector<long lon
     HIP_TRY(hipMemcpy(d, offs.data(), offs.size() * sizeof(long long), hipMemcpyHostToDevice));
     lc.offs = (const long long*)d;
     std::vector<int> lens;
-    const bool multi = choose_passes(PREC_F32, (unsigned long long)L, lens);
+    const bool multi = choose_passes(PREC_F32, (unsigned long long)L, lens, (unsigned long long)lc.count);
     const TileEntry* k1 = (L <= (1 << 30)) ? find_tile(PREC_F32, (int)L, false, false) : nullptr;
     if (k1 && (!multi || lc.count >= min_single)) {
         const TileEntry* k = k1;
@@ -1646,7 +1694,8 @@ std::string tune_path()
     return std::string(h ? h : "/tmp") + "/.cache/gfftx/tune.json";
 }
 
-// One entry per line: "key": [a, b, c]
+// One entry per line: "key": [pass lengths..., 0, nb, wg per pass] (version 1 files hold
+// the pass lengths only, which still load)
 void tune_load()
 {
     const std::string path = tune_path();
@@ -1668,11 +1717,15 @@ void tune_load()
         char* p = lb + 1;
         while (*p && *p != ']') {
             const long x = strtol(p, &p, 10);
-            if (x > 1) v.push_back((int)x);
+            if (x < 0) break;
+            v.push_back((int)x);
             while (*p == ',' || *p == ' ') ++p;
             if (!isdigit((unsigned char)*p) && *p != ']') break;
         }
-        if (v.size() >= 2) g_tune[key] = v;
+        std::vector<int> lens;
+        std::vector<std::pair<int, int>> tiles;
+        split_choice(v, lens, tiles);
+        if (lens.size() >= 2) g_tune[key] = v;
     }
     fclose(f);
 }
@@ -1689,7 +1742,7 @@ void tune_store()
     const std::string tmp = path + ".tmp" + std::to_string((long)getpid());
     FILE* f = fopen(tmp.c_str(), "w");
     if (!f) return;
-    fprintf(f, "{\"version\": 1, \"entries\": {\n");
+    fprintf(f, "{\"version\": 2, \"entries\": {\n");
     size_t i = 0;
     for (const auto& kv : g_tune) {
         fprintf(f, "\"%s\": [", kv.first.c_str());
@@ -1764,7 +1817,33 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
         }
     }
     if (chosen.empty()) {
-        std::vector<std::vector<int>> cands = candidate_passes(p->prec, n, (size_t)env_int("GFFTX_TUNE_CANDIDATES", 5));
+        const std::vector<std::vector<int>> splits =
+            candidate_passes(p->prec, n, (size_t)env_int("GFFTX_TUNE_CANDIDATES", 5), batch, device_cus(device));
+        // each split with its default tiles, then with every other compiled T->T tile of a
+        // length it repeats after the first pass
+        std::vector<std::vector<int>> cands;
+        for (const std::vector<int>& sp : splits) {
+            cands.push_back(sp);
+            std::vector<int> seen;
+            for (size_t i = 1; i < sp.size(); ++i) {
+                const int L = sp[i];
+                if (std::find(seen.begin(), seen.end(), L) != seen.end()) continue;
+                seen.push_back(L);
+                const TileEntry* def = find_tile(p->prec, L, true, true);
+                for (int t = 0; t < kNumTiles; ++t) {
+                    const TileEntry& e = kTiles[t];
+                    if (e.prec != p->prec || e.L != L || !e.loadT || !e.storeT || &e == def) continue;
+                    std::vector<int> c = sp;
+                    c.push_back(0);
+                    for (size_t j = 0; j < sp.size(); ++j) {
+                        const bool alt = j > 0 && sp[j] == L;
+                        c.push_back(alt ? e.nb : 0);
+                        c.push_back(alt ? e.wg : 0);
+                    }
+                    cands.push_back(c);
+                }
+            }
+        }
         if (cands.size() < 2) return GFFTX_SUCCESS;
         DeviceGuard g(device);
         const size_t bytes = (size_t)p->dist * batch * elem_bytes(p->prec);
@@ -1788,9 +1867,15 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
             const int trc = time_plan(cp, a, b, s, 5, &ms);
             gfftx_plan_destroy(cp);
             if (trc) continue;
+            std::vector<int> sl;
+            std::vector<std::pair<int, int>> tl;
+            split_choice(cands[i], sl, tl);
             std::string lens;
-            for (size_t j = 0; j < cands[i].size(); ++j) lens += (j ? "x" : "") + std::to_string(cands[i][j]);
-            char buf[96];
+            for (size_t j = 0; j < sl.size(); ++j) {
+                lens += (j ? "x" : "") + std::to_string(sl[j]);
+                if (j < tl.size() && tl[j].first) lens += "/" + std::to_string(tl[j].first) + "-" + std::to_string(tl[j].second);
+            }
+            char buf[160];
             snprintf(buf, sizeof(buf), "%s{\"split\":\"%s\",\"ms\":%.4f}", i ? "," : "", lens.c_str(), ms);
             timings += buf;
             if (ms < best) {

@@ -237,7 +237,7 @@ def test_autotuner_measures_then_caches(torch_cuda, tmp_path, monkeypatch):
         assert rel_l2(host(d), ref) <= 2e-7
     assert isinstance(d1["tuned"], list) and len(d1["tuned"]) >= 2
     text = cache.read_text()
-    assert '"version": 1' in text and "n=1048576" in text
+    assert '"version": 2' in text and "n=1048576" in text
     with api.Plan(n, api.COMPLEX_FLOAT, api.IDFT, api.OUT_OF_PLACE) as p:
         d2 = json.loads(p.describe())
         assert d2["tuned"] == "cache"

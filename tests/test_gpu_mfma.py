@@ -42,18 +42,20 @@ def test_mfma_forward_inverse(torch_cuda, n, batch):
 
 
 def test_mfma_scaling_edge_cases(torch_cuda):
-    n, batch = 4096, 4
+    n, batch = 4096, 6
     x = inputs.complex_normal(n * batch, 9).astype(np.complex64).reshape(batch, n)
     x[0] = 0                        # all-zero transform (every column max is 0)
     x[1] *= 1e-20                   # tiny magnitudes
     x[2] *= 1e20                    # huge magnitudes (FP16 range needs the scaling)
     x[3, ::7] *= 1e6                # wide dynamic range inside columns
+    x[4] *= 1e-36                   # scale exponent at its clamp (2^126)
+    x[5] *= 1e33                    # near the top of FP32 (the transform still fits)
     with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.IN_PLACE, batch=batch, flags=api.FLAG_SPLIT_MFMA) as p:
         d = dev(torch_cuda, x)
         p.fft(d)
         y = host(d)
     assert np.all(y[0] == 0)
-    for b in (1, 2, 3):
+    for b in (1, 2, 3, 4, 5):
         assert rel_l2(y[b], O.ref_fft(x[b].astype(np.complex128))) <= 2e-7
 
 
