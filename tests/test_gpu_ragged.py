@@ -77,6 +77,20 @@ def test_inverse_round_trip(torch_cuda, monkeypatch, mode):
     check_all(host(d2), xi, lengths, inverse=True)
 
 
+def test_lengths_outside_the_persistent_tiles(torch_cuda):
+    """Lengths the persistent kernel has no tile for (below 16, odd factors of 3 and 5 that
+    are not 3*2^k or 5*2^k, above 65536) run as batched launches of their class."""
+    lengths = [2, 4, 8, 12, 6, 10, 20, 100, 9, 45, 2 ** 17, 12, 3 * 2 ** 15 * 5, 100, 2]
+    x = inputs.complex_normal(sum(lengths), 79).astype(np.complex64)
+    p = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT)
+    d = dev(torch_cuda, x)
+    p.fft(d)
+    check_all(host(d), x, lengths)
+    pi = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.IDFT)
+    pi.fft(d)
+    assert rel_l2(host(d), x) <= 4e-7
+
+
 def test_unsupported_lengths(torch_cuda):
     with pytest.raises(api.UnsupportedTransform):
         api.RaggedPlan([256, 7 * 64], api.COMPLEX_FLOAT)
