@@ -1230,6 +1230,7 @@ constexpr int kRaggedSides = 3;
 
 struct gfftx_ragged_s {
     int device = 0;
+    int prec = PREC_F32;   // FP64 batches run every length class as batched launches
     bool inverse = false;
     uint32_t flags = 0;
     unsigned long long total = 0;          // complex elements of the whole batch
@@ -1301,8 +1302,8 @@ int ragged_long_class(gfftx_ragged_s* r, long long L, const std::vector<long lon
     HIP_TRY(hipMemcpy(d, offs.data(), offs.size() * sizeof(long long), hipMemcpyHostToDevice));
     lc.offs = (const long long*)d;
     std::vector<int> lens;
-    const bool multi = choose_passes(PREC_F32, (unsigned long long)L, lens, (unsigned long long)lc.count);
-    const TileEntry* k1 = (L <= (1 << 30)) ? find_tile(PREC_F32, (int)L, false, false) : nullptr;
+    const bool multi = choose_passes(r->prec, (unsigned long long)L, lens, (unsigned long long)lc.count);
+    const TileEntry* k1 = (L <= (1 << 30)) ? find_tile(r->prec, (int)L, false, false) : nullptr;
     if (k1 && (!multi || lc.count >= min_single)) {
         const TileEntry* k = k1;
         Pass ps;
@@ -1310,14 +1311,14 @@ int ragged_long_class(gfftx_ragged_s* r, long long L, const std::vector<long lon
         ps.items_in_ntrans = true;
         ps.ntrans = lc.count;
         ps.ies = 1;
-        e = upload_stage_table_r(r->allocs, PREC_F32, k->nrad, k->radices, &ps.stage_tw);
+        e = upload_stage_table_r(r->allocs, r->prec, k->nrad, k->radices, &ps.stage_tw);
         if (e) return e;
         lc.passes.push_back(ps);
     } else {
         if (!multi) return fail(GFFTX_ERR_UNSUPPORTED, "ragged batch: no kernel for length %lld", L);
         long long ns = 1;
         for (size_t i = 0; i < lens.size(); ++i) {
-            const TileEntry* k = find_tile(PREC_F32, lens[i], true, i > 0);
+            const TileEntry* k = find_tile(r->prec, lens[i], true, i > 0);
             if (!k) return fail(GFFTX_ERR_UNSUPPORTED, "ragged batch: no pass kernel L=%d", lens[i]);
             Pass ps;
             ps.k = k;
@@ -1327,7 +1328,7 @@ int ragged_long_class(gfftx_ragged_s* r, long long L, const std::vector<long lon
             ps.ies = L / lens[i];
             ps.ns_pass = ns;
             ps.batch = (unsigned)lc.count;
-            e = upload_stage_table_r(r->allocs, PREC_F32, k->nrad, k->radices, &ps.stage_tw);
+            e = upload_stage_table_r(r->allocs, r->prec, k->nrad, k->radices, &ps.stage_tw);
             if (e) return e;
             if (ns > 1) {
                 ps.ptw_m = ns * lens[i];
@@ -1346,13 +1347,13 @@ int ragged_long_class(gfftx_ragged_s* r, long long L, const std::vector<long lon
 // Launches of one long class on stream s, in place on `data`.  Pass i reads the output of
 // pass i-1 and writes scratch S[i % 2], the last pass writes `data` (data -> S0 -> data,
 // data -> S0 -> S1 -> data, ...); the scratch arrays use the data offsets.
-int ragged_run_long(gfftx_ragged_s* r, const gfftx_ragged_s::LongClass& lc, float2* data, int side, hipStream_t s)
+int ragged_run_long(gfftx_ragged_s* r, const gfftx_ragged_s::LongClass& lc, void* data, int side, hipStream_t s)
 {
     const size_t m = lc.passes.size();
     const void* src = data;
     for (size_t i = 0; i < m; ++i) {
         const Pass& ps = lc.passes[i];
-        void* dst = (i + 1 == m) ? (void*)data : r->lscratch[i % 2][side];
+        void* dst = (i + 1 == m) ? data : r->lscratch[i % 2][side];
         TileArgs a;
         memset(&a, 0, sizeof(a));
         a.in = src;
@@ -1454,8 +1455,8 @@ int ragged_build(gfftx_ragged_s* r, const uint64_t* lengths, uint64_t count)
         memset(&rc, 0, sizeof(rc));
         rc.L = L;
         rc.count = (long long)members[c].size();
-        const int single = ragged_single_cfg(L);
-        if (single < 0 && mode == 0) {
+        const int single = r->prec == PREC_F32 ? ragged_single_cfg(L) : -1;
+        if (single < 0 && (mode == 0 || r->prec != PREC_F32)) {
             int e = ragged_long_class(r, L, members[c], env_int("GFFTX_RAGGED_MIN_SINGLE", 0), &nscratch);
             if (e) return e;
             snprintf(buf, sizeof(buf), "%s{\"L\":%lld,\"count\":%lld,\"batched\":[", c ? "," : "", L, rc.count);
@@ -1576,8 +1577,10 @@ int ragged_build(gfftx_ragged_s* r, const uint64_t* lengths, uint64_t count)
     for (const HostItem& h : items) host_items.push_back(h.it);
     for (const HostItem& h : items1) host_items.push_back(h.it);
     void *d_items = nullptr, *d_classes = nullptr, *d_counter = nullptr, *d_scratch = nullptr;
-    int e = dev_alloc_list(r->allocs, &d_items, host_items.size() * sizeof(RaggedItem));
-    if (!e) e = dev_alloc_list(r->allocs, &d_classes, classes.size() * sizeof(RaggedClass));
+    // (at least one element each: every class may run as batched launches, leaving the
+    // persistent kernel without items)
+    int e = dev_alloc_list(r->allocs, &d_items, std::max<size_t>(host_items.size(), 1) * sizeof(RaggedItem));
+    if (!e) e = dev_alloc_list(r->allocs, &d_classes, std::max<size_t>(classes.size(), 1) * sizeof(RaggedClass));
     if (!e) e = dev_alloc_list(r->allocs, &d_counter, 2 * sizeof(unsigned long long));
     if (!e) e = dev_alloc_list(r->allocs, &d_scratch, (size_t)std::max<long long>(scratch, 1) * sizeof(float2));
     if (e) return e;
@@ -1606,7 +1609,7 @@ int ragged_build(gfftx_ragged_s* r, const uint64_t* lengths, uint64_t count)
             HIP_TRY(hipStreamCreateWithFlags(&r->side[k], hipStreamNonBlocking));
             HIP_TRY(hipEventCreateWithFlags(&r->join_ev[k], hipEventDisableTiming));
             for (int b = 0; b < nscratch; ++b) {
-                e = dev_alloc_list(r->allocs, &r->lscratch[b][k], (size_t)off * sizeof(float2));
+                e = dev_alloc_list(r->allocs, &r->lscratch[b][k], (size_t)off * elem_bytes(r->prec));
                 if (e) return e;
             }
         }
@@ -2020,8 +2023,7 @@ int gfftx_ragged_create(gfftx_ragged* plan, const uint64_t* lengths, uint64_t co
     *plan = nullptr;
     if (!lengths || count == 0) return fail(GFFTX_ERR_INVALID_ARG, "empty ragged batch");
     if (count > (1ull << 24)) return fail(GFFTX_ERR_INVALID_ARG, "too many transforms");
-    if (vtype != GFFTX_FLOAT && vtype != GFFTX_COMPLEX_FLOAT)
-        return fail(GFFTX_ERR_UNSUPPORTED, "ragged batches are FP32 (FLOAT / COMPLEX_FLOAT)");
+    if (vtype < 0 || vtype > 3) return fail(GFFTX_ERR_INVALID_ARG, "unknown value type %d", vtype);
     if (ttype != GFFTX_DFT && ttype != GFFTX_IDFT)
         return fail(GFFTX_ERR_UNSUPPORTED, "ragged batches are complex transforms (DFT / IDFT)");
     if (flags & ~(GFFTX_FLAG_GRAPH | GFFTX_FLAG_PROFILE | GFFTX_FLAG_NO_AUTOTUNE))
@@ -2039,6 +2041,7 @@ int gfftx_ragged_create(gfftx_ragged* plan, const uint64_t* lengths, uint64_t co
     gfftx_ragged_s* r = new (std::nothrow) gfftx_ragged_s();
     if (!r) return fail(GFFTX_ERR_ALLOC, "host allocation failed");
     r->device = device;
+    r->prec = (vtype == GFFTX_DOUBLE || vtype == GFFTX_COMPLEX_DOUBLE) ? PREC_F64 : PREC_F32;
     r->inverse = ttype == GFFTX_IDFT;
     r->flags = flags;
     int rc = ragged_build(r, lengths, count);
@@ -2058,7 +2061,7 @@ int gfftx_ragged_exec(gfftx_ragged plan, void* data, void* stream)
     if (!g.ok) return fail(GFFTX_ERR_DEVICE, "cannot select device %d", plan->device);
     hipStream_t s = (hipStream_t)stream;
     if (is_device_ptr(data)) return ragged_launch(plan, data, s);
-    const size_t bytes = (size_t)plan->total * sizeof(float2);
+    const size_t bytes = (size_t)plan->total * elem_bytes(plan->prec);
     if (!plan->stage) {
         int rc = dev_alloc_list(plan->allocs, &plan->stage, bytes);
         if (rc) return rc;

@@ -95,7 +95,25 @@ def test_unsupported_lengths(torch_cuda):
     with pytest.raises(api.UnsupportedTransform):
         api.RaggedPlan([256, 7 * 64], api.COMPLEX_FLOAT)
     with pytest.raises(api.UnsupportedTransform):
-        api.RaggedPlan([256], api.COMPLEX_DOUBLE)
+        api.RaggedPlan([256, 1009], api.COMPLEX_DOUBLE)
+
+
+def test_fp64_batch(torch_cuda):
+    """FP64 ragged batches run every length class as batched launches (relL2 <= 1e-13)."""
+    lengths = [16, 256, 4096, 48, 5120, 65536, 12, 256, 2 ** 17, 3 * 2 ** 10, 16]
+    x = inputs.complex_normal(sum(lengths), 80)
+    pf = api.RaggedPlan(lengths, api.COMPLEX_DOUBLE, api.DFT)
+    pg = api.RaggedPlan(lengths, api.COMPLEX_DOUBLE, api.IDFT, flags=api.FLAG_GRAPH)
+    d = dev(torch_cuda, x)
+    pf.fft(d)
+    check_all(host(d), x, lengths, tol=1e-13)
+    for _ in range(2):
+        g = d.clone()
+        pg.fft(g)
+        assert rel_l2(host(g), x) <= 1e-13
+    h = x.copy()
+    pf.fft(h)  # host array: staged and synchronous
+    assert np.array_equal(h, host(d))
 
 
 @pytest.mark.parametrize("mode", ["0", "1"])
