@@ -23,6 +23,32 @@ template <class C> static hipError_t launch_tile(const TileArgs& a, dim3 grid, h
      &launch_tile<GFFTX_CFG(TYPE, L, NB, WG, LT, ST, __VA_ARGS__)>,                           \
      "tile_" #TYPE "_L" #L "_nb" #NB "_wg" #WG "_" #LT #ST}
 
+// T->T pass tiles with the R2C split fused in (SplitCfg), one per tile shape of the last
+// passes of real forward transforms; NB must be even.
+#define GFFTX_SPLIT_TILE(PREC, TYPE, L, NB, WG, ...)                                                 \
+    {PREC,                                                                                             \
+     L,                                                                                                \
+     NB,                                                                                               \
+     WG,                                                                                               \
+     true,                                                                                             \
+     true,                                                                                             \
+     GFFTX_CFG(TYPE, L, NB, WG, true, true, __VA_ARGS__)::NS,                                        \
+     GFFTX_CFG(TYPE, L, NB, WG, true, true, __VA_ARGS__)::R,                                         \
+     (int)(SplitCfg<GFFTX_CFG(TYPE, L, NB, WG, true, true, __VA_ARGS__)>::LDS_ELEMS * sizeof(V2<TYPE>)), \
+     &launch_tile<SplitCfg<GFFTX_CFG(TYPE, L, NB, WG, true, true, __VA_ARGS__)>>,                    \
+     "tile_" #TYPE "_L" #L "_nb" #NB "_wg" #WG "_split",                                             \
+     true}
+
+#define GFFTX_SPLIT_SET(PREC, TYPE, NB4096, WG4096, NB1024, WG1024, NB1536, WG1536)                 \
+    GFFTX_SPLIT_TILE(PREC, TYPE, 16, 64, 64, 16),                                                     \
+    GFFTX_SPLIT_TILE(PREC, TYPE, 64, 64, 256, 16, 4),                                                 \
+    GFFTX_SPLIT_TILE(PREC, TYPE, 128, 32, 256, 16, 8),                                                \
+    GFFTX_SPLIT_TILE(PREC, TYPE, 256, 16, 256, 16, 16),                                               \
+    GFFTX_SPLIT_TILE(PREC, TYPE, 320, 12, 240, 16, 4, 5),                                             \
+    GFFTX_SPLIT_TILE(PREC, TYPE, 1024, NB1024, WG1024, 16, 16, 4),                                    \
+    GFFTX_SPLIT_TILE(PREC, TYPE, 1536, NB1536, WG1536, 16, 16, 2, 3),                                 \
+    GFFTX_SPLIT_TILE(PREC, TYPE, 4096, NB4096, WG4096, 16, 16, 16)
+
 // Batched single-pass (E->E) tiles.  TPT = L/16 threads per sequence (one radix-16
 // butterfly each), NB sequences per workgroup with NB*TPT close to 256.  Radix-2/4/8/3/5
 // stages spread their butterflies over the same TPT threads.
@@ -117,12 +143,17 @@ TileEntry kTiles[] = {
     GFFTX_TILE(PREC_F32, float, 256, 32, 512, true, true, 16, 16),
     GFFTX_TILE(PREC_F32, float, 256, 32, 256, true, false, 16, 16),
     GFFTX_TILE(PREC_F32, float, 256, 32, 256, true, true, 16, 16),
+    GFFTX_TILE(PREC_F32, float, 1024, 4, 256, true, false, 16, 16, 4),
+    GFFTX_TILE(PREC_F32, float, 1024, 4, 256, true, true, 16, 16, 4),
+    GFFTX_TILE(PREC_F32, float, 1536, 4, 384, true, false, 16, 16, 2, 3),
+    GFFTX_TILE(PREC_F32, float, 1536, 4, 384, true, true, 16, 16, 2, 3),
     GFFTX_TILE(PREC_F32, float, 16, 64, 64, true, true, 16),
     GFFTX_TILE(PREC_F32, float, 64, 64, 256, true, true, 16, 4),
     GFFTX_TILE(PREC_F32, float, 16, 64, 64, true, false, 16),
     GFFTX_TILE(PREC_F32, float, 64, 64, 256, true, false, 16, 4),
     GFFTX_PASS_MIXED(PREC_F32, float),
     GFFTX_PASS_SMALL(PREC_F32, float),
+    GFFTX_SPLIT_SET(PREC_F32, float, 4, 1024, 8, 512, 8, 768),
     GFFTX_BATCH_POW2(PREC_F64, double),
     GFFTX_BATCH_MIXED(PREC_F64, double),
     GFFTX_PASS_SET(PREC_F64, double, 2, 512, 4, 256, 4, 384),
@@ -136,6 +167,7 @@ TileEntry kTiles[] = {
     GFFTX_TILE(PREC_F64, double, 64, 64, 256, true, false, 16, 4),
     GFFTX_PASS_MIXED(PREC_F64, double),
     GFFTX_PASS_SMALL(PREC_F64, double),
+    GFFTX_SPLIT_SET(PREC_F64, double, 2, 512, 4, 256, 4, 384),
 };
 const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 

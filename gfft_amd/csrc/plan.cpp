@@ -102,6 +102,10 @@ struct Pass {
     long long ptw_m = 0;
     int xcd = 0;
     bool items_in_ntrans = false;  // single-pass and MFMA: one sequence per batch item
+    bool split = false;            // SplitCfg tile: the R2C split fused into this last pass
+    double2* sp_lo = nullptr;      // its W_{2N} table
+    double2* sp_hi = nullptr;
+    int sp_shift = 0;
 };
 
 // Batch items [b0, b0 + nb) of a plan; nb == 0 means every item.
@@ -118,12 +122,22 @@ const TileEntry* find_tile(int prec, int L, bool loadT, bool storeT, int want_nb
     const TileEntry* first = nullptr;
     for (int i = 0; i < kNumTiles; ++i) {
         const TileEntry& e = kTiles[i];
-        if (e.prec == prec && e.L == L && e.loadT == loadT && e.storeT == storeT) {
+        if (e.prec == prec && e.L == L && e.loadT == loadT && e.storeT == storeT && !e.split) {
             if ((want_nb || want_wg) && (!want_nb || e.nb == want_nb) && (!want_wg || e.wg == want_wg)) return &e;
             if (!first) first = &e;
         }
     }
     return first;
+}
+
+// The fused-split variant of a T->T pass tile (same length and shape), or null.
+const TileEntry* find_split_tile(const TileEntry* k)
+{
+    for (int i = 0; i < kNumTiles; ++i) {
+        const TileEntry& e = kTiles[i];
+        if (e.split && e.prec == k->prec && e.L == k->L && e.nb == k->nb && e.wg == k->wg) return &e;
+    }
+    return nullptr;
 }
 
 int env_int(const char* name, int dflt)
@@ -187,7 +201,7 @@ std::vector<std::vector<int>> candidate_passes(int prec, unsigned long long n, s
 {
     std::vector<int> lens;
     for (int i = 0; i < kNumTiles; ++i)
-        if (kTiles[i].prec == prec && kTiles[i].loadT && kTiles[i].storeT) lens.push_back(kTiles[i].L);
+        if (kTiles[i].prec == prec && kTiles[i].loadT && kTiles[i].storeT && !kTiles[i].split) lens.push_back(kTiles[i].L);
     std::sort(lens.begin(), lens.end());
     lens.erase(std::unique(lens.begin(), lens.end()), lens.end());
     std::vector<std::pair<double, std::vector<int>>> found;
@@ -343,6 +357,7 @@ struct gfftx_plan_s {
     std::vector<void*> allocs;     // device allocations owned by the plan
     void* scratch[2] = {nullptr, nullptr};
     void* split_tw = nullptr;
+    bool split_fused = false;      // real forward: the split runs inside the last pass
     void* real_work = nullptr;     // IRDFT out-of-place: split runs on a copy of src
     DevBuf stage_in, stage_out;    // device arrays a host-pointer call runs on
     // host-pointer pipeline: two pinned buffers each way, a stream for each copy direction
@@ -577,9 +592,21 @@ int build_c2c_passes(gfftx_plan_s* p)
                 rc = upload_pass_table(p, ps, ns * L);
                 if (rc) return rc;
             }
-                p->passes.push_back(ps);
+            p->passes.push_back(ps);
+            // real forward transform: the last pass also computes the R2C split when its
+            // tile has a fused variant
+            if (i + 1 == lens.size() && p->real && !p->inverse && env_int("GFFTX_FUSED_SPLIT", 1)) {
+                if (const TileEntry* ks = find_split_tile(k)) {
+                    Pass& last = p->passes.back();
+                    last.k = ks;
+                    last.split = true;
+                    rc = upload_root_tables(p->allocs, (long long)(2 * n), &last.sp_lo, &last.sp_hi, &last.sp_shift);
+                    if (rc) return rc;
+                    p->split_fused = true;
+                }
+            }
             snprintf(buf, sizeof(buf), "%s{\"kernel\":\"%s\",\"L\":%d,\"ns\":%lld,\"ntrans\":%lld,\"xcd\":%d}",
-                     i ? "," : "", k->name, L, ns, ps.ntrans, ps.xcd);
+                     i ? "," : "", p->passes.back().k->name, L, ns, ps.ntrans, ps.xcd);
             p->desc += buf;
             ns *= L;
         }
@@ -735,7 +762,17 @@ int run_passes(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, Range 
         a.scale = (p->inverse && i == m - 1) ? 1.0 / (double)p->n : 1.0;
         a.xcd = ps.xcd;
         const long long nb = ps.k->nb;
-        const unsigned ntiles = (unsigned)((a.ntrans + nb - 1) / nb);
+        unsigned ntiles = (unsigned)((a.ntrans + nb - 1) / nb);
+        if (ps.split) {
+            // tile 0: sequences 0 and S/2; then nb/2 pairs (p, S - p), 1 <= p <= (S-1)/2
+            a.sp_lo = ps.sp_lo;
+            a.sp_hi = ps.sp_hi;
+            a.sp_shift = ps.sp_shift;
+            a.split_bins = (p->flags & GFFTX_FLAG_R2C_BINS) ? 1 : 0;
+            a.nfull = (long long)p->n;
+            const long long pairs = (a.ntrans - 1) / 2, h = nb / 2;
+            ntiles = (unsigned)(1 + (pairs + h - 1) / h);
+        }
         dim3 grid(ntiles, ps.items_in_ntrans ? 1u : (unsigned)r.nb);
         std::pair<hipEvent_t, hipEvent_t> ev;
         const bool on = prof_begin(prof, slot0 + i, ps.k->name, s, &ev);
@@ -833,7 +870,7 @@ int exec_device(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, Range
     if (!p->inverse) {
         int rc = run_c2c(p, in, out, s, r);
         if (rc) return rc;
-        return run_split(p, out, s, r);
+        return p->split_fused ? GFFTX_SUCCESS : run_split(p, out, s, r);
     }
     void* work = out;
     if (in != out) {
@@ -1822,24 +1859,25 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
     if (chosen.empty()) {
         const std::vector<std::vector<int>> splits =
             candidate_passes(p->prec, n, (size_t)env_int("GFFTX_TUNE_CANDIDATES", 5), batch, device_cus(device));
-        // each split with its default tiles, then with every other compiled T->T tile of a
-        // length it repeats after the first pass
+        // each split with its default tiles, then with every other compiled tile shape of
+        // its first pass (T->E) and of each length it uses after the first pass (T->T)
         std::vector<std::vector<int>> cands;
         for (const std::vector<int>& sp : splits) {
             cands.push_back(sp);
             std::vector<int> seen;
-            for (size_t i = 1; i < sp.size(); ++i) {
+            for (size_t i = 0; i < sp.size(); ++i) {
                 const int L = sp[i];
-                if (std::find(seen.begin(), seen.end(), L) != seen.end()) continue;
-                seen.push_back(L);
-                const TileEntry* def = find_tile(p->prec, L, true, true);
+                const bool later = i > 0;
+                if (later && std::find(seen.begin(), seen.end(), L) != seen.end()) continue;
+                if (later) seen.push_back(L);
+                const TileEntry* def = find_tile(p->prec, L, true, later);
                 for (int t = 0; t < kNumTiles; ++t) {
                     const TileEntry& e = kTiles[t];
-                    if (e.prec != p->prec || e.L != L || !e.loadT || !e.storeT || &e == def) continue;
+                    if (e.prec != p->prec || e.L != L || !e.loadT || e.storeT != later || e.split || &e == def) continue;
                     std::vector<int> c = sp;
                     c.push_back(0);
                     for (size_t j = 0; j < sp.size(); ++j) {
-                        const bool alt = j > 0 && sp[j] == L;
+                        const bool alt = later ? (j > 0 && sp[j] == L) : j == 0;
                         c.push_back(alt ? e.nb : 0);
                         c.push_back(alt ? e.wg : 0);
                     }
@@ -1863,8 +1901,10 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
         std::string timings = ",\"tuned\":[";
         for (size_t i = 0; i < cands.size(); ++i) {
             gfftx_plan cp = nullptr;
-            if (create_plan_impl(&cp, n, batch, vtype, GFFTX_DFT, GFFTX_OUT_OF_PLACE, device,
-                                 flags & GFFTX_FLAG_R2C_BINS, cands[i], "") != GFFTX_SUCCESS)
+            // the candidates are the complex transforms of the plan (a real plan's bins
+            // layout only widens the buffers, which `bytes` already covers)
+            if (create_plan_impl(&cp, n, batch, vtype, GFFTX_DFT, GFFTX_OUT_OF_PLACE, device, 0, cands[i], "") !=
+                GFFTX_SUCCESS)
                 continue;
             float ms = 0;
             const int trc = time_plan(cp, a, b, s, 5, &ms);

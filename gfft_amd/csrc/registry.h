@@ -23,6 +23,7 @@ struct TileEntry {
     int lds_bytes;
     TileLaunch launch;
     const char* name;
+    bool split;  // SplitCfg: last pass of a real forward transform with the R2C split fused in
 };
 
 extern TileEntry kTiles[];

@@ -43,6 +43,12 @@ struct TileArgs {
     double scale;
     const long long* seq_off;  // optional: element offset of every sequence (E-mode in/out)
     const long long* boff;     // optional: element offset of every batch item (else by*bdist)
+    // fused R2C split (SplitCfg tiles, last pass of a real forward transform)
+    const double2* sp_lo;      // two-level FP64 table of W_{2N}^k
+    const double2* sp_hi;
+    int sp_shift;
+    int split_bins;            // 1: N+1 bins, 0: NR packing
+    long long nfull;           // N, the complex length of the transform
 };
 
 __device__ __forceinline__ long long batch_offset(const TileArgs& a, unsigned by)
@@ -92,8 +98,31 @@ struct TileCfg {
     static constexpr bool DIRECT = (LOADT == STORET);
     static constexpr bool NEED_LDS = (NS > 1) || !DIRECT;
     static constexpr int LDS_ELEMS = NEED_LDS ? NB * TS : 1;
+    static constexpr bool SPLIT = false;
     __device__ static constexpr int lidx(int t, int e) { return t * TS + e + (e >> 4); }
 };
+
+// Last pass of a real forward transform with the R2C split fused in.  The pass computes
+// Z[t + q S] for its S = ntrans sequences t (T->T tile); the split needs Z[k] and Z[N-k],
+// and N - (t + q S) = (S - t) + (L - 1 - q) S for t > 0 (sequence 0 pairs with itself,
+// element q with L - q).  So a tile takes H = NB/2 sequences p and their partners S - p
+// (lanes i and i + H); tile 0 takes the self-paired sequences 0 and, for even S, S/2.
+template <class Base> struct SplitCfg : Base {
+    static_assert(Base::LOADT && Base::STORET, "T->T tiles");
+    static_assert(Base::NB % 2 == 0 && Base::NB >= 2, "sequences pair up inside a tile");
+    static constexpr bool SPLIT = true;
+    static constexpr int LDS_ELEMS = Base::NB * Base::TS;  // the outputs meet in LDS
+};
+
+// Sequence of lane t in tile `tile` under the split pairing (SplitCfg), or -1 for none.
+template <class C> __device__ __forceinline__ long long split_seq(long long S, unsigned tile, int t)
+{
+    constexpr int H = C::NB / 2;
+    if (tile == 0) return t == 0 ? 0 : (t == 1 && S % 2 == 0 && S > 1) ? S / 2 : -1;
+    const long long p = 1 + (long long)(tile - 1) * H + (t % H);
+    if (p > (S - 1) / 2) return -1;
+    return t < H ? p : S - p;
+}
 
 template <class C> struct Ctx {
     using V = typename C::V;
@@ -207,7 +236,7 @@ template <class C, bool IDLE, int S> __device__ __forceinline__ void tile_stage(
 
     if constexpr (!FIRST) __syncthreads();
 
-    if constexpr (LAST && C::DIRECT) {
+    if constexpr (LAST && C::DIRECT && !C::SPLIT) {
         if (x.valid && (!IDLE || x.active)) {
             const TileArgs& a = x.a;
             const unsigned mask = a.conj_out ? 0x80000000u : 0u;
@@ -287,9 +316,16 @@ __device__ __forceinline__ Ctx<C> make_ctx(const TileArgs& a, unsigned tile, uns
         x.t = 0;
         x.jt = 0;
     }
-    x.tg = (long long)tile * C::NB + x.t;
-    x.valid = x.tg < a.ntrans;
-    const long long tl = x.valid ? x.tg : a.ntrans - 1;
+    long long tl;
+    if constexpr (C::SPLIT) {
+        x.tg = split_seq<C>(a.ntrans, tile, x.t);
+        x.valid = x.tg >= 0;
+        tl = x.valid ? x.tg : 0;
+    } else {
+        x.tg = (long long)tile * C::NB + x.t;
+        x.valid = x.tg < a.ntrans;
+        tl = x.valid ? x.tg : a.ntrans - 1;
+    }
     const long long bo = batch_offset(a, by);
     const V* in = reinterpret_cast<const V*>(a.in) + bo;
     V* out = reinterpret_cast<V*>(a.out) + bo;
@@ -338,6 +374,53 @@ template <class C, bool IDLE> __device__ __forceinline__ void tile_finish(const 
     }
 }
 
+// R2C split of a SplitCfg tile from the natural-order outputs in LDS (forward only):
+//   X[k] = (Z[k] + conj Z[N-k]) / 2 - i W_{2N}^k (Z[k] - conj Z[N-k]) / 2,
+// each element written by the thread that stores it, at its own position k = t + q S.
+template <class C, bool IDLE> __device__ __forceinline__ void split_finish(const Ctx<C>& x, unsigned tile, unsigned by)
+{
+    using V = typename C::V;
+    using S_ = decltype(V{}.x);
+    constexpr int H = C::NB / 2;
+    constexpr int PER = C::L * C::NB / C::WG;
+    static_assert(C::L * C::NB % C::WG == 0, "whole elements per thread");
+    const TileArgs& a = x.a;
+    const long long S = a.ntrans, N = a.nfull;
+    const S_ half = (S_)0.5;
+    V* out = reinterpret_cast<V*>(a.out) + batch_offset(a, by);
+    if (IDLE && !x.active) return;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int lin = (int)threadIdx.x + i * C::WG;
+        const int ts = lin % C::NB, q = lin / C::NB;  // T-mode store order
+        const long long tgs = split_seq<C>(S, tile, ts);
+        if (tgs < 0) continue;
+        const int pl = tile == 0 ? ts : (ts < H ? ts + H : ts - H);
+        const int qp = tgs == 0 ? (q == 0 ? 0 : C::L - q) : C::L - 1 - q;
+        const long long k = tgs + (long long)q * S;
+        const V zk = x.lds[C::lidx(ts, q)], zp = x.lds[C::lidx(pl, qp)];
+        if (k == 0) {
+            const S_ x0 = zk.x + zk.y, xn = zk.x - zk.y;
+            if (a.split_bins) {
+                out[0] = V{x0, 0};
+                out[N] = V{xn, 0};
+            } else {
+                out[0] = V{x0, xn};
+            }
+            continue;
+        }
+        const bool lo = k <= N - k;
+        const long long kk = lo ? k : N - k;
+        const V A = lo ? zk : zp, B = lo ? zp : zk;
+        const double2 wl = a.sp_lo[kk & ((1LL << a.sp_shift) - 1)], wh = a.sp_hi[kk >> a.sp_shift];
+        const V w = to_v<V>(dmul(wl, wh));
+        const V h1 = V{half * (A.x + B.x), half * (A.y - B.y)};
+        const V dd = V{half * (A.x - B.x), half * (A.y + B.y)};
+        const V p = cmul(w, V{dd.y, -dd.x});
+        out[k] = lo ? V{h1.x + p.x, h1.y + p.y} : V{h1.x - p.x, p.y - h1.y};
+    }
+}
+
 // One tile, loaded and transformed.  Callable from any kernel whose workgroup has at least
 // C::WG threads (IDLE: more than C::WG, the extra ones only join the barriers) and which
 // provides C::LDS_ELEMS elements of LDS.
@@ -348,7 +431,8 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, unsigned tile, unsi
     FirstIn<C> f;
     load_first<C, IDLE>(x, f);
     stage_loop<C, IDLE, 0>(x, f);
-    tile_finish<C, IDLE>(x, tile, by);
+    if constexpr (C::SPLIT) split_finish<C, IDLE>(x, tile, by);
+    else tile_finish<C, IDLE>(x, tile, by);
 }
 
 template <class C> __global__ __launch_bounds__(C::WG) void tile_kernel(TileArgs a)

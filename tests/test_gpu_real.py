@@ -70,6 +70,45 @@ def test_rdft_irdft(torch_cuda, dtype, n):
         assert np.array_equal(host(s), z)
 
 
+@pytest.mark.parametrize("dtype,n,passes,batch,layout", [
+    (np.complex64, 2 ** 15, "128,256", 3, "nr"), (np.complex64, 49152, "192,256", 1, "bins"),
+    (np.complex64, 46080, "45,1024", 2, "nr"), (np.complex64, 2 ** 18, "1024,256", 1, "bins"),
+    (np.complex64, 3 * 2 ** 19, "1536,1024", 1, "bins"), (np.complex64, 20480, "5,4096", 1, "nr"),
+    (np.complex128, 2 ** 14, "64,256", 2, "nr"), (np.complex128, 40960, "320,128", 1, "bins"),
+    (np.complex128, 46080, "45,1024", 1, "nr"), (np.complex128, 12288, "3,4096", 1, "nr"),
+    (np.complex128, 15360, "15,1024", 1, "bins")])
+def test_fused_split_last_pass(torch_cuda, monkeypatch, dtype, n, passes, batch, layout):
+    """Multi-pass real forward transforms fuse the R2C split into the last pass: its tiles
+    pair sequence t with S - t, S = n / (last pass length), odd (3, 5, 15, 45) and even
+    here.  Against the long-double oracle, against the separate split kernel, and in
+    place."""
+    monkeypatch.setenv("GFFTX_PASSES", passes)
+    z = inputs.complex_normal(n * batch, 300 + n).astype(dtype).reshape(batch, n)
+    flags = api.FLAG_R2C_BINS if layout == "bins" else 0
+    w = n + 1 if layout == "bins" else n
+    print(f"case fused split n={n} batch={batch} {layout} {np.dtype(dtype).name}", flush=True)
+    outs = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("GFFTX_FUSED_SPLIT", fused)
+        with api.Plan(n, VT[dtype], api.RDFT, api.OUT_OF_PLACE, batch=batch, flags=flags) as p:
+            assert ("_split" in p.describe()) == (fused == "1")
+            s = dev(torch_cuda, z)
+            o = torch_cuda.zeros((batch, w), dtype=s.dtype, device="cuda")
+            p.fft(s, o)
+            outs[fused] = host(o)
+    for b in range(batch):
+        ref, X = packed_ref(z[b])
+        want = X if layout == "bins" else ref
+        assert rel_l2(outs["1"][b], want) <= TOL[dtype], b
+    assert rel_l2(outs["1"], outs["0"]) <= TOL[dtype]
+    if layout == "nr":
+        monkeypatch.setenv("GFFTX_FUSED_SPLIT", "1")
+        with api.Plan(n, VT[dtype], api.RDFT, api.IN_PLACE, batch=batch) as p:
+            d = dev(torch_cuda, z)
+            p.fft(d)
+            assert rel_l2(host(d), outs["1"]) <= TOL[dtype]
+
+
 def test_rdft_bins_layout_and_odd_rejected(torch_cuda):
     n = 1024
     z = inputs.complex_normal(n, 9)
