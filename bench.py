@@ -274,6 +274,19 @@ def other_workloads(torch, api):
     c5["mfma_over_plain"] = round(c5["plain"]["us"] / c5["mfma"]["us"], 3)
     res["c5_16384x4096_f32"] = c5
     res["host_pointers"] = host_pointer_workloads(torch, api)
+    # any-N path: a prime length (chirp-z convolution of smooth length m >= 2n-1), FP32
+    n = 1048573  # prime
+    xp = torch.from_numpy(inputs.complex_normal(n, 13).astype(np.complex64)).cuda()
+    op = torch.empty_like(xp)
+    with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE, flags=api.FLAG_PROFILE) as p:
+        ks = _kernel_time_us(p, lambda: p.fft(xp, op))
+        m = json.loads(p.describe())["bluestein"]["m"]
+    us = sum(t for _, t in ks)
+    res["chirp_z_prime_1048573_f32"] = {"us": round(us, 1), "m": m,
+                                        "GFLOPs_5NlogN": round(5 * n * np.log2(n) / us / 1e3, 1),
+                                        "GBps_caller": round(2 * n * 8 / us / 1e3, 1),
+                                        "kernels": len(ks)}
+    del xp, op
     return res
 
 
