@@ -51,7 +51,7 @@ template <class T> __global__ __launch_bounds__(256) void bluestein_pre_kernel(B
     const V2<T>* c = (const V2<T>*)a.chirp;
     const unsigned flip = a.conj ? 0x80000000u : 0u;
     for (unsigned long long b = blockIdx.y; b < (unsigned long long)a.batch; b += gridDim.y) {
-        const V2<T>* x = (const V2<T>*)a.in + b * (unsigned long long)a.dist;
+        const V2<T>* x = (const V2<T>*)a.in + (a.boff ? (unsigned long long)a.boff[b] : b * (unsigned long long)a.dist);
         V2<T>* w = (V2<T>*)a.work + b * (unsigned long long)a.m;
         for (unsigned long long j = blockIdx.x * 256ull + threadIdx.x; j < (unsigned long long)a.m;
              j += (unsigned long long)gridDim.x * 256ull) {
@@ -85,7 +85,7 @@ template <class T> __global__ __launch_bounds__(256) void bluestein_post_kernel(
     const T sc = (T)a.scale;
     for (unsigned long long b = blockIdx.y; b < (unsigned long long)a.batch; b += gridDim.y) {
         const V2<T>* y = (const V2<T>*)a.work + b * (unsigned long long)a.m;
-        V2<T>* o = (V2<T>*)a.out + b * (unsigned long long)a.dist;
+        V2<T>* o = (V2<T>*)a.out + (a.boff ? (unsigned long long)a.boff[b] : b * (unsigned long long)a.dist);
         for (unsigned long long k = blockIdx.x * 256ull + threadIdx.x; k < (unsigned long long)a.n;
              k += (unsigned long long)gridDim.x * 256ull) {
             V2<T> v = cmul(c[k], y[k]);

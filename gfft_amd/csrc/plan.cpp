@@ -803,14 +803,17 @@ int blue_launch(gfftx_plan_s* p, int which, const BluesteinArgs& a, size_t slot,
 
 // Chirp-z transform: pre-multiply into the zero-padded work buffer, forward FFT of length m,
 // multiply by the filter spectrum, inverse FFT, post-multiply into `out`.
-int run_bluestein(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, Range r)
+// `boff` (optional) gives the element offset of every batch item in `in` and `out`; the
+// range must then cover the whole batch.
+int run_bluestein(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, Range r, const long long* boff = nullptr)
 {
     if (!r.nb) r.nb = p->batch;
     const size_t eb = elem_bytes(p->prec);
     BluesteinArgs a;
     memset(&a, 0, sizeof(a));
-    a.in = (const char*)in + (size_t)r.b0 * p->dist * eb;
-    a.out = (char*)out + (size_t)r.b0 * p->dist * eb;
+    a.boff = boff;
+    a.in = (const char*)in + (boff ? 0 : (size_t)r.b0 * p->dist * eb);
+    a.out = (char*)out + (boff ? 0 : (size_t)r.b0 * p->dist * eb);
     a.work = (char*)p->blue_work + (size_t)r.b0 * p->blue_m * eb;
     a.chirp = p->blue_chirp;
     a.filter = p->blue_filter;
@@ -1283,6 +1286,7 @@ struct gfftx_ragged_s {
         long long count = 0;
         const long long* offs = nullptr;  // element offsets of the class's transforms
         std::vector<Pass> passes;         // one pass: single-pass tile over all of them
+        gfftx_plan_s* blue = nullptr;     // length with a prime factor above 5: chirp-z plan
     };
     std::vector<LongClass> longs;
     void* lscratch[2][kRaggedSides] = {};  // per side stream, the size of the whole batch: same offsets
@@ -1338,6 +1342,18 @@ int ragged_long_class(gfftx_ragged_s* r, long long L, const std::vector<long lon
     if (e) return e;
     HIP_TRY(hipMemcpy(d, offs.data(), offs.size() * sizeof(long long), hipMemcpyHostToDevice));
     lc.offs = (const long long*)d;
+    if (!only_235((unsigned long long)L)) {
+        // chirp-z plan for the whole class; its element-wise kernels address the transforms
+        // through the offset table
+        gfftx_plan bp = nullptr;
+        const int cvt = r->prec == PREC_F64 ? GFFTX_COMPLEX_DOUBLE : GFFTX_COMPLEX_FLOAT;
+        e = create_plan(&bp, (uint64_t)L, (uint64_t)lc.count, cvt, r->inverse ? GFFTX_IDFT : GFFTX_DFT,
+                        GFFTX_OUT_OF_PLACE, r->device, r->flags & GFFTX_FLAG_NO_AUTOTUNE);
+        if (e) return e;
+        lc.blue = bp;
+        r->longs.push_back(std::move(lc));
+        return GFFTX_SUCCESS;
+    }
     std::vector<int> lens;
     const bool multi = choose_passes(r->prec, (unsigned long long)L, lens, (unsigned long long)lc.count);
     const TileEntry* k1 = (L <= (1 << 30)) ? find_tile(r->prec, (int)L, false, false) : nullptr;
@@ -1386,6 +1402,7 @@ int ragged_long_class(gfftx_ragged_s* r, long long L, const std::vector<long lon
 // data -> S0 -> S1 -> data, ...); the scratch arrays use the data offsets.
 int ragged_run_long(gfftx_ragged_s* r, const gfftx_ragged_s::LongClass& lc, void* data, int side, hipStream_t s)
 {
+    if (lc.blue) return run_bluestein(lc.blue, data, data, s, Range(), lc.offs);
     const size_t m = lc.passes.size();
     const void* src = data;
     for (size_t i = 0; i < m; ++i) {
@@ -2145,6 +2162,8 @@ int gfftx_ragged_destroy(gfftx_ragged plan)
     if (!plan) return GFFTX_SUCCESS;
     DeviceGuard g(plan->device);
     plan->graphs.clear();
+    for (auto& lc : plan->longs)
+        if (lc.blue) gfftx_plan_destroy(lc.blue);
     for (int k = 0; k < kRaggedSides; ++k) {
         if (plan->side[k]) (void)hipStreamDestroy(plan->side[k]);
         if (plan->join_ev[k]) (void)hipEventDestroy(plan->join_ev[k]);

@@ -69,6 +69,7 @@ struct BluesteinArgs {
     long long n, m, dist, batch;
     int conj;               // inverse transform: conjugate on input and on output
     double scale;           // post: 1 or 1/n
+    const long long* boff;  // optional: element offset of every batch item (else b * dist)
 };
 enum { BLUESTEIN_PRE = 0, BLUESTEIN_MUL = 1, BLUESTEIN_POST = 2, BLUESTEIN_FILTER = 3 };
 hipError_t launch_bluestein(int prec, int which, const BluesteinArgs& a, hipStream_t s);

@@ -116,11 +116,12 @@ int gfftx_plan_profile(gfftx_plan plan, double* total_ms, uint64_t* count, const
                        int* nslots);
 
 /* Ragged batch: `count` transforms of individual lengths stored back to back in one buffer
- * (transform i starts at complex offset sum_{j<i} lengths[j]).  Lengths are any products
- * of 2, 3 and 5 from 2 up to the multi-pass limit, FP32 or FP64.  In FP32, lengths 2^k,
- * 3*2^k, 5*2^k from 16 to 4096 run in one persistent launch; every other length class (all
- * of them in FP64) runs as batched launches on side streams that overlap it; one call (or
- * one hipGraph with GFFTX_FLAG_GRAPH) covers all.
+ * (transform i starts at complex offset sum_{j<i} lengths[j]).  Any length n >= 2 within
+ * the plan limits, FP32 or FP64.  In FP32, lengths 2^k, 3*2^k, 5*2^k from 16 to 4096 run
+ * in one persistent launch; every other length class (all of them in FP64) runs as batched
+ * launches on side streams that overlap it (a class with a prime factor above 5 as a
+ * chirp-z plan over its transforms); one call (or one hipGraph with GFFTX_FLAG_GRAPH)
+ * covers all.
  * No reference counterpart: the nearest is one GFFT object per length from one
  * GenerateTransform<NList,...> factory (src/gfftgen.h:189-233) called in a loop. */
 int gfftx_ragged_create(gfftx_ragged* plan, const uint64_t* lengths, uint64_t count, int vtype,

@@ -93,9 +93,35 @@ def test_lengths_outside_the_persistent_tiles(torch_cuda):
 
 def test_unsupported_lengths(torch_cuda):
     with pytest.raises(api.UnsupportedTransform):
-        api.RaggedPlan([256, 7 * 64], api.COMPLEX_FLOAT)
+        api.RaggedPlan([256, 1], api.COMPLEX_FLOAT)
     with pytest.raises(api.UnsupportedTransform):
-        api.RaggedPlan([256, 1009], api.COMPLEX_DOUBLE)
+        api.RaggedPlan([256, 1], api.COMPLEX_DOUBLE)
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.complex64, 5e-7), (np.complex128, 1e-13)])
+def test_any_length_classes(torch_cuda, dtype, tol):
+    """Lengths with a prime factor above 5 run as chirp-z classes whose element-wise kernels
+    reach the transforms through the offset table; mixed with smooth lengths, both
+    directions, and graph replays."""
+    lengths = [7 * 64, 256, 1009, 7 * 64, 11 * 13, 4096, 1009, 97, 2 ** 17 + 1, 12]
+    x = inputs.complex_normal(sum(lengths), 81).astype(dtype)
+    vt = api.COMPLEX_FLOAT if dtype == np.complex64 else api.COMPLEX_DOUBLE
+    pf = api.RaggedPlan(lengths, vt, api.DFT, flags=api.FLAG_GRAPH)
+    pi = api.RaggedPlan(lengths, vt, api.IDFT)
+    d = dev(torch_cuda, x)
+    pf.fft(d)
+    y = host(d)
+    off = 0
+    for L in lengths:
+        xs = x[off:off + L].astype(np.complex128)
+        ref = np.fft.fft(xs) if L > 4100 else O.dft_ld(xs, 1)
+        assert rel_l2(y[off:off + L], ref) <= tol, L
+        off += L
+    g = dev(torch_cuda, x)
+    pf.fft(g)
+    assert np.array_equal(host(g), y)  # second replay of the captured graph
+    pi.fft(d)
+    assert rel_l2(host(d), x) <= 2 * tol
 
 
 def test_fp64_batch(torch_cuda):
