@@ -97,6 +97,9 @@ int main()
     ok &= check_one<COMPLEX_FLOAT, IDFT, OUT_OF_PLACE, 512>("COMPLEX_FLOAT IDFT oop", 2e-7);
     ok &= check_one<DOUBLE, DFT, OUT_OF_PLACE, 768>("DOUBLE DFT 3*2^8 oop", 1e-13);
     ok &= check_one<FLOAT, DFT, IN_PLACE, 640>("FLOAT DFT 5*2^7 in-place", 2e-7);
+    // a prime length: GFFT's out-of-place path takes any N (chirp-z here)
+    ok &= check_one<DOUBLE, DFT, OUT_OF_PLACE, 1009>("DOUBLE DFT prime oop", 1e-13);
+    ok &= check_one<COMPLEX_FLOAT, IDFT, IN_PLACE, 1001>("COMPLEX_FLOAT IDFT 7*11*13", 5e-7);
 
     // Factory path: objects created by run-time IDs, as src/gfft.cpp:65-67 does.
     typedef GenerateTransform<GenPowerList<4, 10>, DOUBLE, List<DFT, IDFT>, ulong_<1>, HIP<0>, OUT_OF_PLACE> Set;

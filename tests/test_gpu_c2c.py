@@ -320,3 +320,18 @@ def test_digit_reverse_bit_exact(torch_cuda, radix, digits):
     o = torch_cuda.empty_like(s)
     api.digit_reverse(s, o, radix, digits, api.COMPLEX_DOUBLE)
     assert np.array_equal(host(o), O.swap(x, radix, digits))
+
+
+def test_cpp_policy_program(torch_cuda):
+    """tests/cpp/gfft_check (built by __graft_entry__.build()) drives the C++ GFFT surface
+    with the HIP<0> policy: Transform<...>::Instance objects, the GenerateTransform
+    factory by run-time IDs, and the unknown-transform exception."""
+    import os
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "gfft_check")
+    if not os.path.exists(exe):
+        pytest.skip("tests/cpp/gfft_check is not built (run __graft_entry__.build())")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    print(r.stdout[-2000:])
+    assert r.returncode == 0 and "ALL OK" in r.stdout, r.stdout + r.stderr
