@@ -38,6 +38,12 @@ template <typename V, typename S> __device__ __forceinline__ V cscale(V a, S s) 
 #define GFFTX_C5B 0.80901699437494742410229341718281905886  // -cos(4pi/5)
 #define GFFTX_S5A 0.95105651629515357211643933337938214340  // sin(2pi/5)
 #define GFFTX_S5B 0.58778525229247312916870595463907276860  // sin(4pi/5)
+#define GFFTX_C7A 0.62348980185873353052500488400423981063   // cos(2pi/7)
+#define GFFTX_C7B -0.22252093395631440428890256449679475947  // cos(4pi/7)
+#define GFFTX_C7C -0.90096886790241912623610231950744505117  // cos(6pi/7)
+#define GFFTX_S7A 0.78183148246802980870844452667405775023   // sin(2pi/7)
+#define GFFTX_S7B 0.97492791218182360701813168299393121723   // sin(4pi/7)
+#define GFFTX_S7C 0.43388373911755812047576833284835875461   // sin(6pi/7)
 
 template <int R> struct Dft;
 
@@ -93,6 +99,36 @@ template <> struct Dft<5> {
         v[4] = csub(a1, mul_mi(b1));
         v[2] = cadd(a2, mul_mi(b2));
         v[3] = csub(a2, mul_mi(b2));
+    }
+};
+
+template <> struct Dft<7> {
+    // s_k = x_k + x_{7-k}, d_k = x_k - x_{7-k} (k = 1..3);
+    // X_j = x_0 + sum_k cos(2 pi jk/7) s_k - i sum_k sin(2 pi jk/7) d_k, X_{7-j} its partner
+    // with +i; jk mod 7 picks the constant (cos is even, sin odd in jk mod 7).
+    template <typename V> __device__ __forceinline__ static void run(V* v)
+    {
+        using S = decltype(v[0].x);
+        const S c1 = (S)GFFTX_C7A, c2 = (S)GFFTX_C7B, c3 = (S)GFFTX_C7C;
+        const S s1 = (S)GFFTX_S7A, s2 = (S)GFFTX_S7B, s3 = (S)GFFTX_S7C;
+        const V p1 = cadd(v[1], v[6]), m1 = csub(v[1], v[6]);
+        const V p2 = cadd(v[2], v[5]), m2 = csub(v[2], v[5]);
+        const V p3 = cadd(v[3], v[4]), m3 = csub(v[3], v[4]);
+        const V x0 = v[0];
+        const V a1 = V{x0.x + c1 * p1.x + c2 * p2.x + c3 * p3.x, x0.y + c1 * p1.y + c2 * p2.y + c3 * p3.y};
+        const V a2 = V{x0.x + c2 * p1.x + c3 * p2.x + c1 * p3.x, x0.y + c2 * p1.y + c3 * p2.y + c1 * p3.y};
+        const V a3 = V{x0.x + c3 * p1.x + c1 * p2.x + c2 * p3.x, x0.y + c3 * p1.y + c1 * p2.y + c2 * p3.y};
+        // jk mod 7 for j = 2: 2, 4, 6 -> s2, -s3, -s1; for j = 3: 3, 6, 2 -> s3, -s1, s2
+        const V b1 = V{s1 * m1.x + s2 * m2.x + s3 * m3.x, s1 * m1.y + s2 * m2.y + s3 * m3.y};
+        const V b2 = V{s2 * m1.x - s3 * m2.x - s1 * m3.x, s2 * m1.y - s3 * m2.y - s1 * m3.y};
+        const V b3 = V{s3 * m1.x - s1 * m2.x + s2 * m3.x, s3 * m1.y - s1 * m2.y + s2 * m3.y};
+        v[0] = cadd(x0, cadd(p1, cadd(p2, p3)));
+        v[1] = cadd(a1, mul_mi(b1));
+        v[6] = csub(a1, mul_mi(b1));
+        v[2] = cadd(a2, mul_mi(b2));
+        v[5] = csub(a2, mul_mi(b2));
+        v[3] = cadd(a3, mul_mi(b3));
+        v[4] = csub(a3, mul_mi(b3));
     }
 };
 

@@ -126,6 +126,27 @@ template <class C> static hipError_t launch_tile(const TileArgs& a, dim3 grid, h
     GFFTX_TILE(PREC, TYPE, 4096, NB4096, WG4096, true, false, 16, 16, 16),                 \
     GFFTX_TILE(PREC, TYPE, 4096, NB4096, WG4096, true, true, 16, 16, 16)
 
+// Tiles carrying a factor 7 (batched single-pass, and T->E / T->T passes), so that lengths
+// 2^a 3^b 5^c 7^d split into compiled passes.
+#define GFFTX_BATCH_7(PREC, TYPE)                                            \
+    GFFTX_TILE(PREC, TYPE, 7, 64, 64, false, false, 7),                     \
+    GFFTX_TILE(PREC, TYPE, 14, 64, 128, false, false, 7, 2),                \
+    GFFTX_TILE(PREC, TYPE, 28, 64, 256, false, false, 7, 4),                \
+    GFFTX_TILE(PREC, TYPE, 49, 36, 252, false, false, 7, 7),                \
+    GFFTX_TILE(PREC, TYPE, 56, 32, 256, false, false, 7, 8),                \
+    GFFTX_TILE(PREC, TYPE, 112, 36, 252, false, false, 16, 7),              \
+    GFFTX_TILE(PREC, TYPE, 224, 18, 252, false, false, 16, 2, 7),           \
+    GFFTX_TILE(PREC, TYPE, 448, 9, 252, false, false, 16, 4, 7),            \
+    GFFTX_TILE(PREC, TYPE, 896, 4, 224, false, false, 16, 8, 7),            \
+    GFFTX_TILE(PREC, TYPE, 1792, 2, 224, false, false, 16, 16, 7),          \
+    GFFTX_TILE(PREC, TYPE, 3584, 1, 224, false, false, 16, 16, 2, 7)
+#define GFFTX_PASS_7_ONE(PREC, TYPE, ST)                                     \
+    GFFTX_TILE(PREC, TYPE, 7, 256, 256, true, ST, 7),                        \
+    GFFTX_TILE(PREC, TYPE, 49, 36, 252, true, ST, 7, 7),                     \
+    GFFTX_TILE(PREC, TYPE, 112, 36, 252, true, ST, 16, 7),                   \
+    GFFTX_TILE(PREC, TYPE, 448, 9, 252, true, ST, 16, 4, 7)
+#define GFFTX_PASS_7(PREC, TYPE) GFFTX_PASS_7_ONE(PREC, TYPE, false), GFFTX_PASS_7_ONE(PREC, TYPE, true)
+
 // Host table.  Not const: a const namespace-scope table would be emitted into the device
 // image as well; both compilation passes still instantiate every kernel listed here.
 TileEntry kTiles[] = {
@@ -153,6 +174,8 @@ TileEntry kTiles[] = {
     GFFTX_TILE(PREC_F32, float, 64, 64, 256, true, false, 16, 4),
     GFFTX_PASS_MIXED(PREC_F32, float),
     GFFTX_PASS_SMALL(PREC_F32, float),
+    GFFTX_BATCH_7(PREC_F32, float),
+    GFFTX_PASS_7(PREC_F32, float),
     GFFTX_SPLIT_SET(PREC_F32, float, 4, 1024, 8, 512, 8, 768),
     GFFTX_BATCH_POW2(PREC_F64, double),
     GFFTX_BATCH_MIXED(PREC_F64, double),
@@ -167,6 +190,8 @@ TileEntry kTiles[] = {
     GFFTX_TILE(PREC_F64, double, 64, 64, 256, true, false, 16, 4),
     GFFTX_PASS_MIXED(PREC_F64, double),
     GFFTX_PASS_SMALL(PREC_F64, double),
+    GFFTX_BATCH_7(PREC_F64, double),
+    GFFTX_PASS_7(PREC_F64, double),
     GFFTX_SPLIT_SET(PREC_F64, double, 2, 512, 4, 256, 4, 384),
 };
 const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);

@@ -159,13 +159,6 @@ int device_cus(int device)
     return cus;
 }
 
-bool only_235(unsigned long long n)
-{
-    for (unsigned long long p : {2ull, 3ull, 5ull})
-        while (n % p == 0) n /= p;
-    return n == 1;
-}
-
 // Bandwidth (TB/s) a multi-pass tile of length L reaches on MI355X with XCD remapping,
 // measured over N = 2^24 passes; lengths not listed are interpolated by size class.
 double pass_bandwidth(int prec, int L)
@@ -253,6 +246,15 @@ void split_choice(const std::vector<int>& c, std::vector<int>& lens, std::vector
         tiles.push_back({c[i], c[i + 1]});
         i += 2;
     }
+}
+
+// Whether n runs natively: one compiled single-pass tile, or a split into compiled pass
+// tiles (lengths whose factors the tiles carry: 2, 3, 5 and 7).  Any other n runs as a
+// chirp-z convolution.
+bool native_length(int prec, unsigned long long n)
+{
+    if (n <= (1ull << 30) && find_tile(prec, (int)n, false, false)) return true;
+    return !candidate_passes(prec, n, 1).empty();
 }
 
 // Split n into pass lengths available as T-mode tiles (largest first).  An override list
@@ -1242,7 +1244,7 @@ int create_plan_impl(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int
     p->dist = n + ((real && (flags & GFFTX_FLAG_R2C_BINS)) ? 1 : 0);
     p->forced_lens = forced;
     p->tune_note = note;
-    int rc = only_235(n) ? build_c2c_passes(p) : build_bluestein(p);
+    int rc = native_length(p->prec, n) ? build_c2c_passes(p) : build_bluestein(p);
     if (rc == GFFTX_SUCCESS && real) rc = upload_split_table(p);
     if (rc == GFFTX_SUCCESS && real && p->inverse)  // split pass of an out-of-place IRDFT works on a copy
         rc = dev_alloc(p, &p->real_work, (size_t)p->dist * batch * elem_bytes(p->prec));
@@ -1342,7 +1344,7 @@ int ragged_long_class(gfftx_ragged_s* r, long long L, const std::vector<long lon
     if (e) return e;
     HIP_TRY(hipMemcpy(d, offs.data(), offs.size() * sizeof(long long), hipMemcpyHostToDevice));
     lc.offs = (const long long*)d;
-    if (!only_235((unsigned long long)L)) {
+    if (!native_length(r->prec, (unsigned long long)L)) {
         // chirp-z plan for the whole class; its element-wise kernels address the transforms
         // through the offset table
         gfftx_plan bp = nullptr;

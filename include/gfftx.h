@@ -23,7 +23,7 @@
  *     2n+2 scalars on the complex side).
  *
  * Lengths: any n >= 2 (GFFT's out-of-place path takes any N, src/gfftalg.h:459-507).
- * Products of 2, 3 and 5 run as Stockham passes (FP32 up to 2^28, FP64 up to 2^26); any
+ * Products of 2, 3, 5 and 7 run as Stockham passes (FP32 up to 2^28, FP64 up to 2^26); any
  * other n runs as a chirp-z convolution of a 2/3/5-smooth length m >= 2n-1 with the same
  * limits on m.  RDFT/IRDFT need an even n (GFFT's Separate assumes it,
  * src/gfftswap.h:255,279).

@@ -100,6 +100,28 @@ def test_mixed_radix_multi_pass(torch_cuda, dtype, n):
         assert rel_l2(host(o), oracle_fwd(x)) <= TOL[dtype]
 
 
+SEVEN = [7, 14, 28, 49, 56, 112, 224, 448, 896, 1792, 3584, 343, 1680, 7 ** 3 * 2 ** 10, 7 * 2 ** 20]
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n", SEVEN)
+def test_radix7_native(torch_cuda, dtype, n):
+    """Lengths with the factor 7 run on radix-7 butterflies (single-pass and multi-pass),
+    not through the chirp-z path; against the long-double oracle, both directions."""
+    import json
+
+    x = inputs.complex_normal(n, 700 + n).astype(dtype)
+    print(f"case radix7 n={n} dtype={np.dtype(dtype).name}", flush=True)
+    with api.Plan(n, VT[dtype], api.DFT, api.OUT_OF_PLACE) as pf, api.Plan(n, VT[dtype], api.IDFT, api.IN_PLACE) as pi:
+        assert "bluestein" not in json.loads(pf.describe())
+        s = dev(torch_cuda, x)
+        o = torch_cuda.empty_like(s)
+        pf.fft(s, o)
+        assert rel_l2(host(o), oracle_fwd(x)) <= TOL[dtype]
+        pi.fft(s)
+        assert rel_l2(host(s), oracle_inv(x)) <= TOL[dtype]
+
+
 def test_golden_vectors(torch_cuda, golden):
     for n in [2, 4, 8, 16, 32, 64, 128, 256, 768, 1024]:
         x = golden[f"c2c_in_{n}"]
