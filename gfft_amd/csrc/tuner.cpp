@@ -1,0 +1,235 @@
+// tuner.cpp -- plan creation with the measured autotuner and its versioned JSON cache.
+#include "plan_internal.h"
+
+namespace gfftx {
+namespace detail {
+
+// ---------------------------------------------------------------------------------------
+// Autotuner: for a multi-pass plan, time the best few pass splits of the cost model on this
+// device and keep the fastest.  Results are cached per (device, precision, n, batch) in
+// memory and in a versioned JSON file ($GFFTX_TUNE_CACHE, default ~/.cache/gfftx/tune.json).
+std::mutex g_tune_mu;
+std::map<std::string, std::vector<int>> g_tune;
+std::string g_tune_loaded;  // cache file the in-memory entries belong to
+
+std::string tune_path()
+{
+    if (const char* e = getenv("GFFTX_TUNE_CACHE")) return e;
+    const char* h = getenv("HOME");
+    return std::string(h ? h : "/tmp") + "/.cache/gfftx/tune.json";
+}
+
+// One entry per line: "key": [pass lengths..., 0, nb, wg per pass] (version 1 files hold
+// the pass lengths only, which still load)
+void tune_load()
+{
+    const std::string path = tune_path();
+    if (g_tune_loaded == path) return;
+    g_tune_loaded = path;
+    g_tune.clear();
+    FILE* f = fopen(path.c_str(), "r");
+    if (!f) return;
+    char line[512];
+    while (fgets(line, sizeof(line), f)) {
+        char* q1 = strchr(line, '"');
+        if (!q1) continue;
+        char* q2 = strchr(q1 + 1, '"');
+        char* lb = q2 ? strchr(q2, '[') : nullptr;
+        if (!lb) continue;
+        const std::string key(q1 + 1, q2);
+        if (key == "version" || key == "entries") continue;
+        std::vector<int> v;
+        char* p = lb + 1;
+        while (*p && *p != ']') {
+            const long x = strtol(p, &p, 10);
+            if (x < 0) break;
+            v.push_back((int)x);
+            while (*p == ',' || *p == ' ') ++p;
+            if (!isdigit((unsigned char)*p) && *p != ']') break;
+        }
+        std::vector<int> lens;
+        std::vector<std::pair<int, int>> tiles;
+        split_choice(v, lens, tiles);
+        if (lens.size() >= 2) g_tune[key] = v;
+    }
+    fclose(f);
+}
+
+void tune_store()
+{
+    const std::string path = tune_path();
+    const size_t slash = path.find_last_of('/');
+    if (slash != std::string::npos) {
+        std::string dir = path.substr(0, slash);
+        for (size_t i = 1; i <= dir.size(); ++i)
+            if (i == dir.size() || dir[i] == '/') (void)mkdir(dir.substr(0, i).c_str(), 0755);
+    }
+    const std::string tmp = path + ".tmp" + std::to_string((long)getpid());
+    FILE* f = fopen(tmp.c_str(), "w");
+    if (!f) return;
+    fprintf(f, "{\"version\": 2, \"entries\": {\n");
+    size_t i = 0;
+    for (const auto& kv : g_tune) {
+        fprintf(f, "\"%s\": [", kv.first.c_str());
+        for (size_t j = 0; j < kv.second.size(); ++j) fprintf(f, "%s%d", j ? ", " : "", kv.second[j]);
+        fprintf(f, "]%s\n", ++i < g_tune.size() ? "," : "");
+    }
+    fprintf(f, "}}\n");
+    fclose(f);
+    (void)rename(tmp.c_str(), path.c_str());
+}
+
+std::string tune_key(int device, int prec, uint64_t n, uint64_t batch)
+{
+    hipDeviceProp_t prop;
+    std::string dev = "dev";
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) dev = std::string(prop.gcnArchName) + ":" + std::to_string(prop.multiProcessorCount) + "cu";
+    else (void)hipGetLastError();
+    return dev + "|" + (prec == PREC_F64 ? "f64" : "f32") + "|n=" + std::to_string(n) + "|b=" + std::to_string(batch);
+}
+
+// Median of `reps` timed executions (ms) of a plan on scratch buffers.
+int time_plan(gfftx_plan_s* p, void* a, void* b, hipStream_t s, int reps, float* ms_out)
+{
+    for (int i = 0; i < 2; ++i) {
+        int rc = exec_device(p, a, b, s);
+        if (rc) return rc;
+    }
+    std::vector<float> t;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    for (int i = 0; i < reps; ++i) {
+        (void)hipEventRecord(e0, s);
+        int rc = exec_device(p, a, b, s);
+        (void)hipEventRecord(e1, s);
+        if (rc) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return rc;
+        }
+        (void)hipEventSynchronize(e1);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        t.push_back(ms);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::sort(t.begin(), t.end());
+    *ms_out = t[t.size() / 2];
+    return GFFTX_SUCCESS;
+}
+
+int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttype, int place, int device,
+                uint32_t flags)
+{
+    int rc = create_plan_impl(out, n, batch, vtype, ttype, place, device, flags, {}, "");
+    if (rc != GFFTX_SUCCESS) return rc;
+    gfftx_plan_s* p = *out;
+    if (p->passes.size() < 2 || (flags & GFFTX_FLAG_NO_AUTOTUNE) || env_int("GFFTX_AUTOTUNE", 1) == 0 ||
+        getenv("GFFTX_PASSES"))
+        return GFFTX_SUCCESS;
+    const std::string key = tune_key(device, p->prec, n, batch);
+    std::vector<int> chosen;
+    std::string note;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        tune_load();
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            chosen = it->second;
+            note = ",\"tuned\":\"cache\"";
+        }
+    }
+    if (chosen.empty()) {
+        const std::vector<std::vector<int>> splits =
+            candidate_passes(p->prec, n, (size_t)env_int("GFFTX_TUNE_CANDIDATES", 5), batch, device_cus(device));
+        // each split with its default tiles, then with every other compiled tile shape of
+        // its first pass (T->E) and of each length it uses after the first pass (T->T)
+        std::vector<std::vector<int>> cands;
+        for (const std::vector<int>& sp : splits) {
+            cands.push_back(sp);
+            std::vector<int> seen;
+            for (size_t i = 0; i < sp.size(); ++i) {
+                const int L = sp[i];
+                const bool later = i > 0;
+                if (later && std::find(seen.begin(), seen.end(), L) != seen.end()) continue;
+                if (later) seen.push_back(L);
+                const TileEntry* def = find_tile(p->prec, L, true, later);
+                for (int t = 0; t < kNumTiles; ++t) {
+                    const TileEntry& e = kTiles[t];
+                    if (e.prec != p->prec || e.L != L || !e.loadT || e.storeT != later || e.split || &e == def) continue;
+                    std::vector<int> c = sp;
+                    c.push_back(0);
+                    for (size_t j = 0; j < sp.size(); ++j) {
+                        const bool alt = later ? (j > 0 && sp[j] == L) : j == 0;
+                        c.push_back(alt ? e.nb : 0);
+                        c.push_back(alt ? e.wg : 0);
+                    }
+                    cands.push_back(c);
+                }
+            }
+        }
+        if (cands.size() < 2) return GFFTX_SUCCESS;
+        DeviceGuard g(device);
+        const size_t bytes = (size_t)p->dist * batch * elem_bytes(p->prec);
+        void *a = nullptr, *b = nullptr;
+        if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            if (a) (void)hipFree(a);
+            return GFFTX_SUCCESS;  // no room to tune: keep the cost-model plan
+        }
+        hipStream_t s = nullptr;
+        (void)hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        (void)hipMemsetAsync(a, 0, bytes, s);
+        float best = 1e30f;
+        std::string timings = ",\"tuned\":[";
+        for (size_t i = 0; i < cands.size(); ++i) {
+            gfftx_plan cp = nullptr;
+            // the candidates are the complex transforms of the plan (a real plan's bins
+            // layout only widens the buffers, which `bytes` already covers)
+            if (create_plan_impl(&cp, n, batch, vtype, GFFTX_DFT, GFFTX_OUT_OF_PLACE, device, 0, cands[i], "") !=
+                GFFTX_SUCCESS)
+                continue;
+            float ms = 0;
+            const int trc = time_plan(cp, a, b, s, 5, &ms);
+            gfftx_plan_destroy(cp);
+            if (trc) continue;
+            std::vector<int> sl;
+            std::vector<std::pair<int, int>> tl;
+            split_choice(cands[i], sl, tl);
+            std::string lens;
+            for (size_t j = 0; j < sl.size(); ++j) {
+                lens += (j ? "x" : "") + std::to_string(sl[j]);
+                if (j < tl.size() && tl[j].first) lens += "/" + std::to_string(tl[j].first) + "-" + std::to_string(tl[j].second);
+            }
+            char buf[160];
+            snprintf(buf, sizeof(buf), "%s{\"split\":\"%s\",\"ms\":%.4f}", i ? "," : "", lens.c_str(), ms);
+            timings += buf;
+            if (ms < best) {
+                best = ms;
+                chosen = cands[i];
+            }
+        }
+        timings += "]";
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+        (void)hipFree(a);
+        (void)hipFree(b);
+        (void)hipGetLastError();
+        if (chosen.empty()) return GFFTX_SUCCESS;
+        note = timings;
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[key] = chosen;
+        tune_store();
+    }
+    gfftx_plan q = nullptr;
+    rc = create_plan_impl(&q, n, batch, vtype, ttype, place, device, flags, chosen, note);
+    if (rc != GFFTX_SUCCESS) return GFFTX_SUCCESS;  // keep the cost-model plan
+    gfftx_plan_destroy(p);
+    *out = q;
+    return GFFTX_SUCCESS;
+}
+
+}  // namespace detail
+}  // namespace gfftx
