@@ -136,6 +136,7 @@ template <class C> struct Ctx {
     unsigned ons;        // output element stride (the store rule's Ns)
     bool valid;          // sequence exists (loads are clamped to a valid one regardless)
     bool active;         // thread takes part (threads beyond C::WG idle through the barriers)
+    long long bo;        // element offset of the batch item
     TileArgs a;
 };
 
@@ -146,13 +147,6 @@ template <> __device__ __forceinline__ double2 to_v<double2>(double2 w) { return
 __device__ __forceinline__ double2 dmul(double2 a, double2 b)
 {
     return double2{__builtin_fma(a.x, b.x, -(a.y * b.y)), __builtin_fma(a.x, b.y, a.y * b.x)};
-}
-
-__device__ __forceinline__ double2 pass_root(const TileArgs& a, long long m)
-{
-    const double2 lo = a.ptw_lo[m & ((1LL << a.ptw_shift) - 1)];
-    const double2 hi = a.ptw_hi[m >> a.ptw_shift];
-    return dmul(lo, hi);
 }
 
 // Conjugation as a sign-bit flip of the imaginary part: mask is 0 or the sign bit.
@@ -167,9 +161,15 @@ __device__ __forceinline__ double2 flip_im(double2 u, unsigned mask)
     return u;
 }
 
-// First-stage inputs of one thread (raw loads, conjugated for inverse transforms).
+// First-stage inputs of one thread, as loaded (the first stage conjugates them for inverse
+// transforms, so that nothing waits on a load before the stage needs its value).
 template <class C> struct FirstIn {
     typename C::V v[C::nbf(0)][C::R[0]];
+    // pass twiddle of a later pass: the two table halves of W^{kp*j} per group and of the
+    // step W^{kp*L/R}, fetched with the inputs (rot: the sequence has kp != 0)
+    double2 blo[C::nbf(0)], bhi[C::nbf(0)];
+    double2 slo, shi;
+    bool rot;
 };
 
 template <class C, bool IDLE> __device__ __forceinline__ void load_first(const Ctx<C>& x, FirstIn<C>& f)
@@ -179,13 +179,29 @@ template <class C, bool IDLE> __device__ __forceinline__ void load_first(const C
     constexpr int B = C::nbf(0);
     constexpr int LR = C::L / R;
     constexpr bool GUARD = (B * C::TPT > LR);
-    const unsigned mask = x.a.conj_in ? 0x80000000u : 0u;
 #pragma unroll
     for (int b = 0; b < B; ++b) {
         const int j = x.jt + b * C::TPT;
         if ((IDLE && !x.active) || (GUARD && j >= LR)) continue;
 #pragma unroll
-        for (int r = 0; r < R; ++r) f.v[b][r] = flip_im(x.in[(unsigned)(j + r * LR) * x.ies], mask);
+        for (int r = 0; r < R; ++r) f.v[b][r] = x.in[(unsigned)(j + r * LR) * x.ies];
+    }
+    const TileArgs& a = x.a;
+    f.rot = false;
+    if (a.ptw_lo != nullptr) {
+        const long long kp = x.tg % a.ns_pass;
+        f.rot = kp != 0;
+        if (f.rot) {
+            const long long mask = (1LL << a.ptw_shift) - 1;
+            f.slo = a.ptw_lo[(kp * LR) & mask];
+            f.shi = a.ptw_hi[(kp * LR) >> a.ptw_shift];
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const long long m = kp * (x.jt + b * C::TPT);
+                f.blo[b] = a.ptw_lo[m & mask];
+                f.bhi[b] = a.ptw_hi[m >> a.ptw_shift];
+            }
+        }
     }
 }
 
@@ -207,20 +223,18 @@ template <class C, bool IDLE, int S> __device__ __forceinline__ void tile_stage(
         if ((IDLE && !x.active) || (GUARD && j >= LR)) continue;
         if constexpr (FIRST) {
             const TileArgs& a = x.a;
+            const unsigned mask = a.conj_in ? 0x80000000u : 0u;
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[b][r] = f.v[b][r];
-            if (a.ptw_lo != nullptr) {
+            for (int r = 0; r < R; ++r) v[b][r] = flip_im(f.v[b][r], mask);
+            if (f.rot) {
                 // W_M^{kp*e}, e = j + r*LR: base W^{kp*j} times step^r, step = W^{kp*LR}
-                const long long kp = x.tg % a.ns_pass;
-                if (kp != 0) {
-                    const double2 step = pass_root(a, kp * LR);
-                    double2 w = pass_root(a, kp * j);
-                    v[b][0] = cmul(v[b][0], to_v<V>(w));
+                const double2 step = dmul(f.slo, f.shi);
+                double2 w = dmul(f.blo[b], f.bhi[b]);
+                v[b][0] = cmul(v[b][0], to_v<V>(w));
 #pragma unroll
-                    for (int r = 1; r < R; ++r) {
-                        w = dmul(w, step);
-                        v[b][r] = cmul(v[b][r], to_v<V>(w));
-                    }
+                for (int r = 1; r < R; ++r) {
+                    w = dmul(w, step);
+                    v[b][r] = cmul(v[b][r], to_v<V>(w));
                 }
             }
         } else {
@@ -329,6 +343,7 @@ __device__ __forceinline__ Ctx<C> make_ctx(const TileArgs& a, unsigned tile, uns
     const long long bo = batch_offset(a, by);
     const V* in = reinterpret_cast<const V*>(a.in) + bo;
     V* out = reinterpret_cast<V*>(a.out) + bo;
+    x.bo = bo;
     x.in = in + (a.seq_off ? a.seq_off[tl] : tl * a.its);
     x.out = out + out_base<C>(a, tl);
     x.ies = (unsigned)a.ies;
@@ -350,19 +365,61 @@ template <class C, bool IDLE> __device__ __forceinline__ void tile_finish(const 
         const auto sc = (decltype(V{}.x))a.scale;
         const bool scale = a.scale != 1.0;
         const int tid = threadIdx.x;
-        V* out = reinterpret_cast<V*>(a.out) + batch_offset(a, by);
+        V* out = reinterpret_cast<V*>(a.out) + x.bo;
+        if (!C::STORET && ns == 1 && a.seq_off == nullptr) {
+            // Contiguous sequences (the first pass of a multi-pass transform, batched E-mode
+            // output): the tile's NB*L outputs form one contiguous run, and element lin of the
+            // run is element q = lin % L of sequence lin / L.  The address needs no per-element
+            // store rule, so the stores issue back to back.
+            V* o = out + (long long)tile * C::NB * C::L;
+            const long long left = a.ntrans - (long long)tile * C::NB;
+            const int nvalid = left < C::NB ? (int)left : C::NB;
+            if ((!IDLE || x.active) && nvalid == C::NB) {
+                // full tile: no guard, so the LDS reads issue together
+                V u[PER];
+#pragma unroll
+                for (int i = 0; i < PER; ++i) {
+                    const int lin = tid + i * C::WG;
+                    u[i] = x.lds[C::lidx(lin / C::L, lin % C::L)];
+                }
+#pragma unroll
+                for (int i = 0; i < PER; ++i) o[tid + i * C::WG] = flip_im(scale ? cscale(u[i], sc) : u[i], mask);
+            } else if (!IDLE || x.active) {
+#pragma unroll
+                for (int i = 0; i < PER; ++i) {
+                    const int lin = tid + i * C::WG;
+                    const int ts = lin / C::L, q = lin % C::L;
+                    if (ts < nvalid) {
+                        V u = x.lds[C::lidx(ts, q)];
+                        if (scale) u = cscale(u, sc);
+                        o[lin] = flip_im(u, mask);
+                    }
+                }
+            }
+            return;
+        }
+        if constexpr (C::STORET) {
+            // WG is a multiple of NB, so element lin = tid + i*WG of the tile belongs to
+            // sequence tid % NB for every i: the sequence's output base is computed once.
+            const int ts = tid % C::NB;
+            const long long tgs = (long long)tile * C::NB + ts;
+            if ((!IDLE || x.active) && tgs < a.ntrans) {
+                V* o = out + out_base<C>(a, tgs);
+#pragma unroll
+                for (int i = 0; i < PER; ++i) {
+                    const int q = (tid + i * C::WG) / C::NB;
+                    V u = x.lds[C::lidx(ts, q)];
+                    if (scale) u = cscale(u, sc);
+                    o[(unsigned)q * ns] = flip_im(u, mask);
+                }
+            }
+            return;
+        }
         if (!IDLE || x.active) {
 #pragma unroll
             for (int i = 0; i < PER; ++i) {
                 const int lin = tid + i * C::WG;
-                int ts, q;
-                if (C::STORET) {
-                    ts = lin % C::NB;
-                    q = lin / C::NB;
-                } else {
-                    ts = lin / C::L;
-                    q = lin % C::L;
-                }
+                const int ts = lin / C::L, q = lin % C::L;
                 const long long tgs = (long long)tile * C::NB + ts;
                 if (tgs < a.ntrans) {
                     V u = x.lds[C::lidx(ts, q)];
@@ -387,7 +444,7 @@ template <class C, bool IDLE> __device__ __forceinline__ void split_finish(const
     const TileArgs& a = x.a;
     const long long S = a.ntrans, N = a.nfull;
     const S_ half = (S_)0.5;
-    V* out = reinterpret_cast<V*>(a.out) + batch_offset(a, by);
+    V* out = reinterpret_cast<V*>(a.out) + x.bo;
     if (IDLE && !x.active) return;
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
