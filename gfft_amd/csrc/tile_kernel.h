@@ -149,6 +149,15 @@ __device__ __forceinline__ double2 dmul(double2 a, double2 b)
     return double2{__builtin_fma(a.x, b.x, -(a.y * b.y)), __builtin_fma(a.x, b.y, a.y * b.x)};
 }
 
+// Non-temporal load of one complex element (the cache lines are marked for early eviction).
+template <class V> __device__ __forceinline__ V nt_load(const V* p)
+{
+    using E = decltype(V{}.x);
+    typedef E vec __attribute__((ext_vector_type(2)));
+    const vec r = __builtin_nontemporal_load(reinterpret_cast<const vec*>(p));
+    return V{r.x, r.y};
+}
+
 // Conjugation as a sign-bit flip of the imaginary part: mask is 0 or the sign bit.
 __device__ __forceinline__ float2 flip_im(float2 u, unsigned mask)
 {
@@ -172,6 +181,12 @@ template <class C> struct FirstIn {
     bool rot;
 };
 
+// The input of a pass is dead once read (a scratch array, or the caller's input that the
+// transform consumes), while its output is what the next pass reads.  Loading the input
+// non-temporally lets the caches (L2 and the 256 MB Infinity Cache) keep the output instead:
+// a 2^24 FP32 round trip runs about 30% faster.  Used where the tile's loads cover whole
+// 128-byte lines (T mode: NB adjacent sequences per row; E mode: NB whole sequences), so
+// that no other tile needs a line this one evicts early.
 template <class C, bool IDLE> __device__ __forceinline__ void load_first(const Ctx<C>& x, FirstIn<C>& f)
 {
     using V = typename C::V;
@@ -179,12 +194,16 @@ template <class C, bool IDLE> __device__ __forceinline__ void load_first(const C
     constexpr int B = C::nbf(0);
     constexpr int LR = C::L / R;
     constexpr bool GUARD = (B * C::TPT > LR);
+    constexpr bool NT = (C::LOADT ? C::NB : C::NB * C::L) * (int)sizeof(V) >= 128;
 #pragma unroll
     for (int b = 0; b < B; ++b) {
         const int j = x.jt + b * C::TPT;
         if ((IDLE && !x.active) || (GUARD && j >= LR)) continue;
 #pragma unroll
-        for (int r = 0; r < R; ++r) f.v[b][r] = x.in[(unsigned)(j + r * LR) * x.ies];
+        for (int r = 0; r < R; ++r) {
+            const V* p = &x.in[(unsigned)(j + r * LR) * x.ies];
+            f.v[b][r] = NT ? nt_load(p) : *p;
+        }
     }
     const TileArgs& a = x.a;
     f.rot = false;
