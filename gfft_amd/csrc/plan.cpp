@@ -82,7 +82,9 @@ int build_c2c_passes(gfftx_plan_s* p)
         p->desc += "]}";
         return GFFTX_SUCCESS;
     }
-    const TileEntry* single = n <= (1ull << 30) ? find_tile(p->prec, (int)n, false, false) : nullptr;
+    // a forced pass split (autotuner candidates) overrides the single-pass tile
+    const TileEntry* single =
+        n <= (1ull << 30) && p->forced_lens.empty() ? find_tile(p->prec, (int)n, false, false) : nullptr;
     if (single) {
         const TileEntry* k = single;
         Pass ps;
@@ -114,7 +116,7 @@ int build_c2c_passes(gfftx_plan_s* p)
             if (!k) return fail(GFFTX_ERR_UNSUPPORTED, "no pass kernel L=%d", L);
             Pass ps;
             ps.k = k;
-            ps.xcd = env_int("GFFTX_XCD", 1);
+            ps.xcd = xcd_remap(k, p->prec);
             ps.ntrans = (long long)(n / L);
             ps.its = 1;
             ps.ies = (long long)(n / L);

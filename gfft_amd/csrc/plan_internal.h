@@ -84,6 +84,7 @@ double pass_occupancy(int prec, int L, bool first, unsigned long long n, unsigne
 std::vector<std::vector<int>> candidate_passes(int prec, unsigned long long n, size_t k, unsigned long long batch = 1,
                                                int cus = 256);
 void split_choice(const std::vector<int>& c, std::vector<int>& lens, std::vector<std::pair<int, int>>& tiles);
+int xcd_remap(const TileEntry* k, int prec);
 bool native_length(int prec, unsigned long long n);
 bool choose_passes(int prec, unsigned long long n, std::vector<int>& out, unsigned long long batch = 1, int cus = 256);
 unsigned long long bluestein_max_m(int prec);
@@ -278,6 +279,8 @@ int exec_host(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, bool di
 // --- tuner.cpp: plan creation with the measured autotuner
 int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttype, int place, int device,
                 uint32_t flags);
+bool tune_length_class(int device, int prec, long long L, long long count, bool inverse, bool* single,
+                       std::vector<int>* lens);
 
 }  // namespace detail
 }  // namespace gfftx

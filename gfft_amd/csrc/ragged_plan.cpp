@@ -25,12 +25,12 @@ int ragged_pass1_cfg(long long P)
 
 // A length class above the persistent kernel's tiles: the transforms' offsets on the
 // device, and the pass list that computes all of them in one batched launch per pass.
-// A single-pass tile (one workgroup per transform, 16 bytes of HBM traffic per point)
-// is used when one exists and the class has at least `min_single` transforms; otherwise
-// multi-pass tiles (many workgroups per transform, 32 bytes per point).  The classes run
-// concurrently, so the lower traffic wins even for classes too small to fill the device
-// (measured on the 4096-transform batch).  `nscratch` is raised to the scratch arrays the
-// pass list needs.
+// The autotuner times the class's options (tune_length_class): a single-pass tile (one
+// workgroup per transform, 16 bytes of HBM traffic per point) against the best pass splits
+// (many workgroups per transform, 32 bytes per point).  Without a measurement the rule is:
+// the single-pass tile when one exists and the class has at least `min_single` transforms,
+// else the cost model's split.  `nscratch` is raised to the scratch arrays the pass list
+// needs.
 int ragged_long_class(gfftx_ragged_s* r, long long L, const std::vector<long long>& offs, long long min_single,
                       int* nscratch)
 {
@@ -55,9 +55,24 @@ int ragged_long_class(gfftx_ragged_s* r, long long L, const std::vector<long lon
         return GFFTX_SUCCESS;
     }
     std::vector<int> lens;
-    const bool multi = choose_passes(r->prec, (unsigned long long)L, lens, (unsigned long long)lc.count);
+    bool multi = choose_passes(r->prec, (unsigned long long)L, lens, (unsigned long long)lc.count);
     const TileEntry* k1 = (L <= (1 << 30)) ? find_tile(r->prec, (int)L, false, false) : nullptr;
-    if (k1 && (!multi || lc.count >= min_single)) {
+    bool use_single = k1 && (!multi || lc.count >= min_single);
+    // measured choice between the single-pass tile and the pass splits (unless the caller
+    // turned autotuning off or fixed the rule with GFFTX_RAGGED_MIN_SINGLE)
+    bool t_single = false;
+    std::vector<int> t_lens;
+    if (!(r->flags & GFFTX_FLAG_NO_AUTOTUNE) && !getenv("GFFTX_RAGGED_MIN_SINGLE") && !getenv("GFFTX_PASSES") &&
+        tune_length_class(r->device, r->prec, L, lc.count, r->inverse, &t_single, &t_lens)) {
+        if (t_single && k1) {
+            use_single = true;
+        } else if (!t_single && t_lens.size() >= 2) {
+            use_single = false;
+            multi = true;
+            lens = t_lens;
+        }
+    }
+    if (use_single) {
         const TileEntry* k = k1;
         Pass ps;
         ps.k = k;
@@ -75,7 +90,7 @@ int ragged_long_class(gfftx_ragged_s* r, long long L, const std::vector<long lon
             if (!k) return fail(GFFTX_ERR_UNSUPPORTED, "ragged batch: no pass kernel L=%d", lens[i]);
             Pass ps;
             ps.k = k;
-            ps.xcd = 1;
+            ps.xcd = xcd_remap(k, r->prec);
             ps.ntrans = L / lens[i];
             ps.its = 1;
             ps.ies = L / lens[i];

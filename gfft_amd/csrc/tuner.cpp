@@ -231,5 +231,76 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
     return GFFTX_SUCCESS;
 }
 
+// Plan choice for one length class of a ragged batch (count transforms of length L):
+// the single-pass tile, when one exists, against the cost model's best three pass splits,
+// each timed on a contiguous batch of the class's size.  Returns false when nothing was
+// measured (autotuning off, or no buffers), and the caller keeps its built-in rule.  Cached
+// like plans, under the class's own key; a single-pass choice is stored as [L, 1].
+bool tune_length_class(int device, int prec, long long L, long long count, bool inverse, bool* single,
+                       std::vector<int>* lens)
+{
+    if (env_int("GFFTX_AUTOTUNE", 1) == 0) return false;
+    const bool has_single = L <= (1 << 30) && find_tile(prec, (int)L, false, false) != nullptr;
+    const std::vector<std::vector<int>> splits = candidate_passes(prec, (unsigned long long)L, 3,
+                                                                  (unsigned long long)count, device_cus(device));
+    if ((has_single ? 1 : 0) + splits.size() < 2) return false;
+    const std::string key = tune_key(device, prec, (uint64_t)L, (uint64_t)count) + "|class";
+    std::vector<int> chosen;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        tune_load();
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) chosen = it->second;
+    }
+    if (chosen.empty()) {
+        std::vector<std::vector<int>> cands;
+        if (has_single) cands.push_back({});
+        for (const std::vector<int>& sp : splits) cands.push_back(sp);
+        DeviceGuard g(device);
+        const size_t bytes = (size_t)L * (size_t)count * elem_bytes(prec);
+        void *a = nullptr, *b = nullptr;
+        if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            if (a) (void)hipFree(a);
+            return false;
+        }
+        hipStream_t s = nullptr;
+        (void)hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        (void)hipMemsetAsync(a, 0, bytes, s);
+        const int cvt = prec == PREC_F64 ? GFFTX_COMPLEX_DOUBLE : GFFTX_COMPLEX_FLOAT;
+        float best = 1e30f;
+        bool found = false;
+        for (const std::vector<int>& c : cands) {
+            gfftx_plan cp = nullptr;
+            if (create_plan_impl(&cp, (uint64_t)L, (uint64_t)count, cvt, inverse ? GFFTX_IDFT : GFFTX_DFT,
+                                 GFFTX_OUT_OF_PLACE, device, GFFTX_FLAG_NO_AUTOTUNE, c, "") != GFFTX_SUCCESS)
+                continue;
+            float ms = 0;
+            const int trc = time_plan(cp, a, b, s, 5, &ms);
+            gfftx_plan_destroy(cp);
+            if (trc) continue;
+            if (ms < best) {
+                best = ms;
+                chosen = c.empty() ? std::vector<int>{(int)L, 1} : c;
+                found = true;
+            }
+        }
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+        (void)hipFree(a);
+        (void)hipFree(b);
+        (void)hipGetLastError();
+        if (!found) return false;
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[key] = chosen;
+        tune_store();
+    }
+    std::vector<std::pair<int, int>> tiles;
+    split_choice(chosen, *lens, tiles);
+    *single = lens->size() == 2 && (*lens)[1] == 1;
+    if (*single) lens->clear();
+    return true;
+}
+
 }  // namespace detail
 }  // namespace gfftx
