@@ -459,19 +459,28 @@ template <class C, bool IDLE> __device__ __forceinline__ void split_finish(const
     using S_ = decltype(V{}.x);
     constexpr int H = C::NB / 2;
     constexpr int PER = C::L * C::NB / C::WG;
+    constexpr int QS = C::WG / C::NB;  // q step between a thread's elements
     static_assert(C::L * C::NB % C::WG == 0, "whole elements per thread");
     const TileArgs& a = x.a;
     const long long S = a.ntrans, N = a.nfull;
     const S_ half = (S_)0.5;
     V* out = reinterpret_cast<V*>(a.out) + x.bo;
     if (IDLE && !x.active) return;
+    // WG is a multiple of NB: element i of this thread is element q = q0 + i*QS of sequence ts
+    // (T-mode store order), so its k = tgs + q*S steps by QS*S and the twiddles W_{2N}^k form
+    // a geometric sequence: an FP64 recurrence from two table roots, and W_{2N}^{N-k} =
+    // -conj(W_{2N}^k) for the mirrored half.
+    const int ts = (int)threadIdx.x % C::NB, q0 = (int)threadIdx.x / C::NB;
+    const long long tgs = split_seq<C>(S, tile, ts);
+    if (tgs < 0) return;
+    const int pl = tile == 0 ? ts : (ts < H ? ts + H : ts - H);
+    const long long mask = (1LL << a.sp_shift) - 1;
+    const long long k0 = tgs + (long long)q0 * S, kst = (long long)QS * S;
+    double2 w = dmul(a.sp_lo[k0 & mask], a.sp_hi[k0 >> a.sp_shift]);
+    const double2 step = dmul(a.sp_lo[kst & mask], a.sp_hi[kst >> a.sp_shift]);
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-        const int lin = (int)threadIdx.x + i * C::WG;
-        const int ts = lin % C::NB, q = lin / C::NB;  // T-mode store order
-        const long long tgs = split_seq<C>(S, tile, ts);
-        if (tgs < 0) continue;
-        const int pl = tile == 0 ? ts : (ts < H ? ts + H : ts - H);
+        const int q = q0 + i * QS;
         const int qp = tgs == 0 ? (q == 0 ? 0 : C::L - q) : C::L - 1 - q;
         const long long k = tgs + (long long)q * S;
         const V zk = x.lds[C::lidx(ts, q)], zp = x.lds[C::lidx(pl, qp)];
@@ -483,17 +492,16 @@ template <class C, bool IDLE> __device__ __forceinline__ void split_finish(const
             } else {
                 out[0] = V{x0, xn};
             }
-            continue;
+        } else {
+            const bool lo = k <= N - k;
+            const V A = lo ? zk : zp, B = lo ? zp : zk;
+            const V wv = to_v<V>(lo ? w : double2{-w.x, w.y});  // W^k, or W^{N-k} = -conj(W^k)
+            const V h1 = V{half * (A.x + B.x), half * (A.y - B.y)};
+            const V dd = V{half * (A.x - B.x), half * (A.y + B.y)};
+            const V p = cmul(wv, V{dd.y, -dd.x});
+            out[k] = lo ? V{h1.x + p.x, h1.y + p.y} : V{h1.x - p.x, p.y - h1.y};
         }
-        const bool lo = k <= N - k;
-        const long long kk = lo ? k : N - k;
-        const V A = lo ? zk : zp, B = lo ? zp : zk;
-        const double2 wl = a.sp_lo[kk & ((1LL << a.sp_shift) - 1)], wh = a.sp_hi[kk >> a.sp_shift];
-        const V w = to_v<V>(dmul(wl, wh));
-        const V h1 = V{half * (A.x + B.x), half * (A.y - B.y)};
-        const V dd = V{half * (A.x - B.x), half * (A.y + B.y)};
-        const V p = cmul(w, V{dd.y, -dd.x});
-        out[k] = lo ? V{h1.x + p.x, h1.y + p.y} : V{h1.x - p.x, p.y - h1.y};
+        w = dmul(w, step);
     }
 }
 
