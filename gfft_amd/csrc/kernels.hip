@@ -168,6 +168,16 @@ TileEntry kTiles[] = {
     GFFTX_TILE(PREC_F32, float, 1024, 4, 256, true, true, 16, 16, 4),
     GFFTX_TILE(PREC_F32, float, 1536, 4, 384, true, false, 16, 16, 2, 3),
     GFFTX_TILE(PREC_F32, float, 1536, 4, 384, true, true, 16, 16, 2, 3),
+    // narrow tiles: more workgroups for transforms of a few million points, whose passes
+    // are latency-bound (the autotuner weighs them against the wider rows above)
+    GFFTX_TILE(PREC_F32, float, 1536, 2, 192, true, false, 16, 16, 2, 3),
+    GFFTX_TILE(PREC_F32, float, 1536, 2, 192, true, true, 16, 16, 2, 3),
+    GFFTX_TILE(PREC_F32, float, 1024, 2, 128, true, false, 16, 16, 4),
+    GFFTX_TILE(PREC_F32, float, 1024, 2, 128, true, true, 16, 16, 4),
+    GFFTX_SPLIT_TILE(PREC_F32, float, 1536, 2, 192, 16, 16, 2, 3),
+    GFFTX_SPLIT_TILE(PREC_F32, float, 1536, 4, 384, 16, 16, 2, 3),
+    GFFTX_SPLIT_TILE(PREC_F32, float, 1024, 4, 256, 16, 16, 4),
+    GFFTX_SPLIT_TILE(PREC_F32, float, 1024, 2, 128, 16, 16, 4),
     GFFTX_TILE(PREC_F32, float, 16, 64, 64, true, true, 16),
     GFFTX_TILE(PREC_F32, float, 64, 64, 256, true, true, 16, 4),
     GFFTX_TILE(PREC_F32, float, 16, 64, 64, true, false, 16),

@@ -3,6 +3,8 @@
 GFFT's RDFT maps 2n reals to n complex values in NR packing [X_0, X_n, X_1 .. X_{n-1}]
 (src/gfftswap.h:239-287); IRDFT inverts it including the 1/n of Backward.  The bins layout
 (n+1 complex bins) is the engine's extra format for the 3*2^20 workload."""
+import json
+
 import numpy as np
 import pytest
 
@@ -140,3 +142,31 @@ def test_c3_config_r2c_3x2p20(torch_cuda):
     # peaks at the seeded bins with amplitude A*N/2
     for f, a in zip(freqs, amps):
         assert abs(abs(y[f]) - a * nr / 2) < 1e-3 * a * nr / 2 + 50
+
+
+@pytest.mark.parametrize("tnb", [2, 4])
+@pytest.mark.parametrize("passes", ["1536,1024", "1024,1536"])
+def test_narrow_pass_tiles(torch_cuda, monkeypatch, tnb, passes):
+    """Narrow pass tiles (2 or 4 sequences per workgroup, chosen with GFFTX_TNB) at the
+    C3 shape n = 3*2^19: the real forward transform with the split fused into the last
+    pass, against the packed reference; the complex forward and inverse against numpy."""
+    monkeypatch.setenv("GFFTX_PASSES", passes)
+    monkeypatch.setenv("GFFTX_TNB", str(tnb))
+    n = 3 * 2 ** 19
+    z = inputs.complex_normal(n, 70 + tnb).astype(np.complex64)
+    print(f"case narrow tiles nb={tnb} passes={passes}", flush=True)
+    with api.Plan(n, api.FLOAT, api.RDFT, api.OUT_OF_PLACE) as p:
+        desc = json.loads(p.describe())
+        assert all(f"_nb{tnb}_" in ps["kernel"] for ps in desc["passes"]), desc
+        assert desc["passes"][-1]["kernel"].endswith("_split"), desc
+        s = dev(torch_cuda, z)
+        o = torch_cuda.empty_like(s)
+        p.fft(s, o)
+        ref, _ = packed_ref(z)
+        assert rel_l2(host(o), ref) <= 2e-7
+    x = z.astype(np.complex128)
+    for tt, want in ((api.DFT, np.fft.fft(x)), (api.IDFT, np.fft.ifft(x))):
+        with api.Plan(n, api.COMPLEX_FLOAT, tt, api.IN_PLACE) as p:
+            d = dev(torch_cuda, z)
+            p.fft(d)
+            assert rel_l2(host(d), want) <= 2e-7
