@@ -178,6 +178,16 @@ TileEntry kTiles[] = {
     GFFTX_SPLIT_TILE(PREC_F32, float, 1536, 4, 384, 16, 16, 2, 3),
     GFFTX_SPLIT_TILE(PREC_F32, float, 1024, 4, 256, 16, 16, 4),
     GFFTX_SPLIT_TILE(PREC_F32, float, 1024, 2, 128, 16, 16, 4),
+    // eight points per thread (radix-8 stages): twice the waves of the radix-16 tiles
+    GFFTX_TILE(PREC_F32, float, 1536, 4, 768, true, false, 8, 8, 8, 3),
+    GFFTX_TILE(PREC_F32, float, 1536, 4, 768, true, true, 8, 8, 8, 3),
+    GFFTX_SPLIT_TILE(PREC_F32, float, 1536, 4, 768, 8, 8, 8, 3),
+    GFFTX_TILE(PREC_F32, float, 1024, 8, 1024, true, false, 8, 8, 8, 2),
+    GFFTX_TILE(PREC_F32, float, 1024, 8, 1024, true, true, 8, 8, 8, 2),
+    GFFTX_SPLIT_TILE(PREC_F32, float, 1024, 8, 1024, 8, 8, 8, 2),
+    GFFTX_TILE(PREC_F32, float, 1024, 4, 512, true, false, 8, 8, 8, 2),
+    GFFTX_TILE(PREC_F32, float, 1024, 4, 512, true, true, 8, 8, 8, 2),
+    GFFTX_SPLIT_TILE(PREC_F32, float, 1024, 4, 512, 8, 8, 8, 2),
     GFFTX_TILE(PREC_F32, float, 16, 64, 64, true, true, 16),
     GFFTX_TILE(PREC_F32, float, 64, 64, 256, true, true, 16, 4),
     GFFTX_TILE(PREC_F32, float, 16, 64, 64, true, false, 16),

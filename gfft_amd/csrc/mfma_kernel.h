@@ -99,10 +99,11 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
         const long long tg = t0 + t;
         float2 v[4];
         if constexpr (FIRST) {
-            // loads past the last sequence are clamped to it; their results are not stored
+            // loads past the last sequence are clamped to it; their results are not stored.
+            // The input is read once, so it is loaded non-temporally (whole sequences).
             const float2* src = a.in + (tg < a.ntrans ? tg : a.ntrans - 1) * C::L;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) v[i] = flip_im(src[(unsigned)(j + (4 * g + i) * C::COLS)], cmask_in);
+            for (int i = 0; i < 4; ++i) v[i] = flip_im(nt_load(&src[(unsigned)(j + (4 * g + i) * C::COLS)]), cmask_in);
         } else {
 #pragma unroll
             for (int i = 0; i < 4; ++i) v[i] = lds[C::lidx(t, j + (4 * g + i) * C::COLS)];

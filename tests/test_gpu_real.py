@@ -170,3 +170,32 @@ def test_narrow_pass_tiles(torch_cuda, monkeypatch, tnb, passes):
             d = dev(torch_cuda, z)
             p.fft(d)
             assert rel_l2(host(d), want) <= 2e-7
+
+
+@pytest.mark.parametrize("n,passes,tnb,twg", [
+    (1536 * 1536, "1536,1536", 4, 768), (1024 * 1024, "1024,1024", 8, 1024),
+    (1024 * 1024, "1024,1024", 4, 512)])
+def test_radix8_pass_tiles(torch_cuda, monkeypatch, n, passes, tnb, twg):
+    """Pass tiles with eight points per thread (radix-8 stages), forced through GFFTX_TNB and
+    GFFTX_TWG: the real forward transform with the fused split, and the complex forward and
+    inverse, against numpy."""
+    monkeypatch.setenv("GFFTX_PASSES", passes)
+    monkeypatch.setenv("GFFTX_TNB", str(tnb))
+    monkeypatch.setenv("GFFTX_TWG", str(twg))
+    z = inputs.complex_normal(n, 80 + tnb).astype(np.complex64)
+    print(f"case radix-8 tiles n={n} nb={tnb} wg={twg}", flush=True)
+    with api.Plan(n, api.FLOAT, api.RDFT, api.OUT_OF_PLACE) as p:
+        desc = json.loads(p.describe())
+        assert all(f"_nb{tnb}_wg{twg}_" in ps["kernel"] for ps in desc["passes"]), desc
+        assert desc["passes"][-1]["kernel"].endswith("_split"), desc
+        s = dev(torch_cuda, z)
+        o = torch_cuda.empty_like(s)
+        p.fft(s, o)
+        ref, _ = packed_ref(z)
+        assert rel_l2(host(o), ref) <= 2e-7
+    x = z.astype(np.complex128)
+    for tt, want in ((api.DFT, np.fft.fft(x)), (api.IDFT, np.fft.ifft(x))):
+        with api.Plan(n, api.COMPLEX_FLOAT, tt, api.IN_PLACE) as p:
+            d = dev(torch_cuda, z)
+            p.fft(d)
+            assert rel_l2(host(d), want) <= 2e-7
