@@ -153,6 +153,9 @@ TileEntry kTiles[] = {
     GFFTX_BATCH_POW2(PREC_F32, float),
     GFFTX_BATCH_MIXED(PREC_F32, float),
     GFFTX_TILE(PREC_F32, float, 10240, 1, 640, false, false, 16, 16, 8, 5),
+    // 4096 points with eight per thread (radix-8 stages, 512 threads): the autotuner weighs
+    // it against the radix-16 tile for batched transforms
+    GFFTX_TILE(PREC_F32, float, 4096, 1, 512, false, false, 8, 8, 8, 8),
     GFFTX_TILE(PREC_F32, float, 12288, 1, 768, false, false, 16, 16, 16, 3),
     GFFTX_TILE(PREC_F32, float, 16384, 1, 1024, false, false, 16, 16, 16, 4),
     GFFTX_PASS_SET(PREC_F32, float, 4, 1024, 8, 512, 8, 768),

@@ -82,9 +82,16 @@ int build_c2c_passes(gfftx_plan_s* p)
         p->desc += "]}";
         return GFFTX_SUCCESS;
     }
-    // a forced pass split (autotuner candidates) overrides the single-pass tile
-    const TileEntry* single =
-        n <= (1ull << 30) && p->forced_lens.empty() ? find_tile(p->prec, (int)n, false, false) : nullptr;
+    // a forced choice (autotuner candidates, tuning cache): [n, 1, 0, nb, wg] names a
+    // single-pass tile; any other list is a pass split, which overrides the single-pass tile
+    const bool forced_single = p->forced_lens.size() >= 2 && p->forced_lens[0] == (int)n && p->forced_lens[1] == 1;
+    const TileEntry* single = nullptr;
+    if (n <= (1ull << 30) && (p->forced_lens.empty() || forced_single)) {
+        std::vector<int> fl;
+        std::vector<std::pair<int, int>> ft;
+        split_choice(p->forced_lens, fl, ft);
+        single = find_tile(p->prec, (int)n, false, false, ft.empty() ? 0 : ft[0].first, ft.empty() ? 0 : ft[0].second);
+    }
     if (single) {
         const TileEntry* k = single;
         Pass ps;

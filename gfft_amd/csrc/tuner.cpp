@@ -126,8 +126,11 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
     int rc = create_plan_impl(out, n, batch, vtype, ttype, place, device, flags, {}, "");
     if (rc != GFFTX_SUCCESS) return rc;
     gfftx_plan_s* p = *out;
-    if (p->passes.size() < 2 || (flags & GFFTX_FLAG_NO_AUTOTUNE) || env_int("GFFTX_AUTOTUNE", 1) == 0 ||
-        getenv("GFFTX_PASSES"))
+    // a batched single-pass plan is tuned over the compiled tiles of its length (shape and
+    // radices), a multi-pass plan over pass splits and tile shapes
+    const bool single_pass = p->passes.size() == 1 && p->passes[0].k && p->passes[0].items_in_ntrans;
+    if ((p->passes.size() < 2 && !single_pass) || (flags & GFFTX_FLAG_NO_AUTOTUNE) ||
+        env_int("GFFTX_AUTOTUNE", 1) == 0 || getenv("GFFTX_PASSES"))
         return GFFTX_SUCCESS;
     const std::string key = tune_key(device, p->prec, n, batch);
     std::vector<int> chosen;
@@ -142,8 +145,9 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
         }
     }
     if (chosen.empty()) {
-        const std::vector<std::vector<int>> splits =
-            candidate_passes(p->prec, n, (size_t)env_int("GFFTX_TUNE_CANDIDATES", 5), batch, device_cus(device));
+        std::vector<std::vector<int>> splits;
+        if (!single_pass)
+            splits = candidate_passes(p->prec, n, (size_t)env_int("GFFTX_TUNE_CANDIDATES", 5), batch, device_cus(device));
         // each split with its default tiles, then with every other compiled tile shape of
         // its first pass (T->E) and of each length it uses after the first pass (T->T)
         std::vector<std::vector<int>> cands;
@@ -170,6 +174,12 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
                 }
             }
         }
+        if (single_pass)
+            for (int t = 0; t < kNumTiles; ++t) {
+                const TileEntry& e = kTiles[t];
+                if (e.prec == p->prec && (uint64_t)e.L == n && !e.loadT && !e.storeT && !e.split)
+                    cands.push_back({(int)n, 1, 0, e.nb, e.wg});
+            }
         if (cands.size() < 2) return GFFTX_SUCCESS;
         DeviceGuard g(device);
         const size_t bytes = (size_t)p->dist * batch * elem_bytes(p->prec);
@@ -199,9 +209,13 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
             std::vector<std::pair<int, int>> tl;
             split_choice(cands[i], sl, tl);
             std::string lens;
-            for (size_t j = 0; j < sl.size(); ++j) {
-                lens += (j ? "x" : "") + std::to_string(sl[j]);
-                if (j < tl.size() && tl[j].first) lens += "/" + std::to_string(tl[j].first) + "-" + std::to_string(tl[j].second);
+            if (single_pass) {
+                lens = "single/" + std::to_string(tl[0].first) + "-" + std::to_string(tl[0].second);
+            } else {
+                for (size_t j = 0; j < sl.size(); ++j) {
+                    lens += (j ? "x" : "") + std::to_string(sl[j]);
+                    if (j < tl.size() && tl[j].first) lens += "/" + std::to_string(tl[j].first) + "-" + std::to_string(tl[j].second);
+                }
             }
             char buf[160];
             snprintf(buf, sizeof(buf), "%s{\"split\":\"%s\",\"ms\":%.4f}", i ? "," : "", lens.c_str(), ms);

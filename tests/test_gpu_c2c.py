@@ -271,6 +271,43 @@ def test_autotuner_measures_then_caches(torch_cuda, tmp_path, monkeypatch):
         assert "tuned" not in json.loads(p.describe())
 
 
+def test_single_pass_tiles_tuned_and_cached(torch_cuda, tmp_path, monkeypatch):
+    """A batched single-pass plan whose length has several compiled tiles (4096 FP32: radix
+    16x16x16 with 256 threads, radix 8x8x8x8 with 512) times them and caches the choice as
+    [n, 1, 0, nb, wg].  Each tile, forced through its own cache file, matches the oracle, and
+    the two agree."""
+    import json
+
+    n, batch = 4096, 64
+    x = np.stack([inputs.complex_normal(n, 300 + b) for b in range(batch)]).astype(np.complex64)
+    ref = np.stack([oracle_fwd(r) for r in x])
+    cache = tmp_path / "tune.json"
+    monkeypatch.setenv("GFFTX_TUNE_CACHE", str(cache))
+    with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.IN_PLACE, batch=batch) as p:
+        desc = json.loads(p.describe())
+    assert isinstance(desc["tuned"], list) and len(desc["tuned"]) >= 2, desc
+    assert all(t["split"].startswith("single/") for t in desc["tuned"])
+    lines = cache.read_text().splitlines()
+    key = [ln for ln in lines if f"n={n}|b={batch}" in ln]
+    assert len(key) == 1, lines
+    prefix = key[0].split("[")[0]
+    outs = {}
+    for nb, wg in ((1, 256), (1, 512)):
+        forced = tmp_path / f"tune_{wg}.json"
+        forced.write_text('{"version": 2, "entries": {\n' + prefix + f"[{n}, 1, 0, {nb}, {wg}]\n" + "}}\n")
+        monkeypatch.setenv("GFFTX_TUNE_CACHE", str(forced))
+        print(f"case single-pass tile nb={nb} wg={wg}", flush=True)
+        with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.IN_PLACE, batch=batch) as p:
+            desc = json.loads(p.describe())
+            assert desc["tuned"] == "cache" and f"_nb{nb}_wg{wg}_" in desc["passes"][0]["kernel"], desc
+            d = dev(torch_cuda, x)
+            p.fft(d)
+            outs[wg] = host(d)
+        for b in range(batch):
+            assert rel_l2(outs[wg][b], ref[b]) <= 2e-7, (wg, b)
+    assert rel_l2(outs[256], outs[512]) <= 2e-7
+
+
 def test_out_of_place_alias_rejected(torch_cuda):
     with api.Plan(256, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE) as p:
         d = torch_cuda.zeros(256, dtype=torch_cuda.complex64, device="cuda")
