@@ -156,6 +156,10 @@ TileEntry kTiles[] = {
     // 4096 points with eight per thread (radix-8 stages, 512 threads): the autotuner weighs
     // it against the radix-16 tile for batched transforms
     GFFTX_TILE(PREC_F32, float, 4096, 1, 512, false, false, 8, 8, 8, 8),
+    GFFTX_TILE(PREC_F32, float, 2048, 2, 512, false, false, 8, 8, 8, 4),
+    GFFTX_TILE(PREC_F32, float, 5120, 1, 640, false, false, 8, 8, 8, 2, 5),
+    GFFTX_TILE(PREC_F32, float, 6144, 1, 768, false, false, 8, 8, 8, 4, 3),
+    GFFTX_TILE(PREC_F32, float, 8192, 1, 1024, false, false, 8, 8, 8, 16),
     GFFTX_TILE(PREC_F32, float, 12288, 1, 768, false, false, 16, 16, 16, 3),
     GFFTX_TILE(PREC_F32, float, 16384, 1, 1024, false, false, 16, 16, 16, 4),
     GFFTX_PASS_SET(PREC_F32, float, 4, 1024, 8, 512, 8, 768),

@@ -57,15 +57,17 @@ int ragged_long_class(gfftx_ragged_s* r, long long L, const std::vector<long lon
     std::vector<int> lens;
     bool multi = choose_passes(r->prec, (unsigned long long)L, lens, (unsigned long long)lc.count);
     const TileEntry* k1 = (L <= (1 << 30)) ? find_tile(r->prec, (int)L, false, false) : nullptr;
+    int t_nb = 0, t_wg = 0;
     bool use_single = k1 && (!multi || lc.count >= min_single);
     // measured choice between the single-pass tile and the pass splits (unless the caller
     // turned autotuning off or fixed the rule with GFFTX_RAGGED_MIN_SINGLE)
     bool t_single = false;
     std::vector<int> t_lens;
     if (!(r->flags & GFFTX_FLAG_NO_AUTOTUNE) && !getenv("GFFTX_RAGGED_MIN_SINGLE") && !getenv("GFFTX_PASSES") &&
-        tune_length_class(r->device, r->prec, L, lc.count, r->inverse, &t_single, &t_lens)) {
+        tune_length_class(r->device, r->prec, L, lc.count, r->inverse, &t_single, &t_lens, &t_nb, &t_wg)) {
         if (t_single && k1) {
             use_single = true;
+            k1 = find_tile(r->prec, (int)L, false, false, t_nb, t_wg);
         } else if (!t_single && t_lens.size() >= 2) {
             use_single = false;
             multi = true;

@@ -245,13 +245,14 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
     return GFFTX_SUCCESS;
 }
 
-// Plan choice for one length class of a ragged batch (count transforms of length L):
-// the single-pass tile, when one exists, against the cost model's best three pass splits,
+// Plan choice for one length class of a ragged batch (count transforms of length L): every
+// compiled single-pass tile of the length against the cost model's best three pass splits,
 // each timed on a contiguous batch of the class's size.  Returns false when nothing was
 // measured (autotuning off, or no buffers), and the caller keeps its built-in rule.  Cached
-// like plans, under the class's own key; a single-pass choice is stored as [L, 1].
+// like plans, under the class's own key; a single-pass choice is stored as [L, 1, 0, nb, wg]
+// and returned with *single set and the tile's shape in *nb, *wg.
 bool tune_length_class(int device, int prec, long long L, long long count, bool inverse, bool* single,
-                       std::vector<int>* lens)
+                       std::vector<int>* lens, int* nb, int* wg)
 {
     if (env_int("GFFTX_AUTOTUNE", 1) == 0) return false;
     const bool has_single = L <= (1 << 30) && find_tile(prec, (int)L, false, false) != nullptr;
@@ -268,7 +269,10 @@ bool tune_length_class(int device, int prec, long long L, long long count, bool 
     }
     if (chosen.empty()) {
         std::vector<std::vector<int>> cands;
-        if (has_single) cands.push_back({});
+        for (int t = 0; has_single && t < kNumTiles; ++t) {
+            const TileEntry& e = kTiles[t];
+            if (e.prec == prec && e.L == L && !e.loadT && !e.storeT && !e.split) cands.push_back({(int)L, 1, 0, e.nb, e.wg});
+        }
         for (const std::vector<int>& sp : splits) cands.push_back(sp);
         DeviceGuard g(device);
         const size_t bytes = (size_t)L * (size_t)count * elem_bytes(prec);
@@ -295,7 +299,7 @@ bool tune_length_class(int device, int prec, long long L, long long count, bool 
             if (trc) continue;
             if (ms < best) {
                 best = ms;
-                chosen = c.empty() ? std::vector<int>{(int)L, 1} : c;
+                chosen = c;
                 found = true;
             }
         }
@@ -312,6 +316,8 @@ bool tune_length_class(int device, int prec, long long L, long long count, bool 
     std::vector<std::pair<int, int>> tiles;
     split_choice(chosen, *lens, tiles);
     *single = lens->size() == 2 && (*lens)[1] == 1;
+    *nb = *single && !tiles.empty() ? tiles[0].first : 0;
+    *wg = *single && !tiles.empty() ? tiles[0].second : 0;
     if (*single) lens->clear();
     return true;
 }

@@ -271,14 +271,16 @@ def test_autotuner_measures_then_caches(torch_cuda, tmp_path, monkeypatch):
         assert "tuned" not in json.loads(p.describe())
 
 
-def test_single_pass_tiles_tuned_and_cached(torch_cuda, tmp_path, monkeypatch):
-    """A batched single-pass plan whose length has several compiled tiles (4096 FP32: radix
-    16x16x16 with 256 threads, radix 8x8x8x8 with 512) times them and caches the choice as
+@pytest.mark.parametrize("n,batch,tiles", [
+    (4096, 64, ((1, 256), (1, 512))), (2048, 40, ((2, 256), (2, 512))), (5120, 24, ((1, 320), (1, 640))),
+    (6144, 24, ((1, 384), (1, 768))), (8192, 16, ((1, 512), (1, 1024)))])
+def test_single_pass_tiles_tuned_and_cached(torch_cuda, tmp_path, monkeypatch, n, batch, tiles):
+    """A batched single-pass plan whose length has several compiled tiles (radix-16 stages,
+    and radix-8 stages with twice the threads) times them and caches the choice as
     [n, 1, 0, nb, wg].  Each tile, forced through its own cache file, matches the oracle, and
     the two agree."""
     import json
 
-    n, batch = 4096, 64
     x = np.stack([inputs.complex_normal(n, 300 + b) for b in range(batch)]).astype(np.complex64)
     ref = np.stack([oracle_fwd(r) for r in x])
     cache = tmp_path / "tune.json"
@@ -292,11 +294,11 @@ def test_single_pass_tiles_tuned_and_cached(torch_cuda, tmp_path, monkeypatch):
     assert len(key) == 1, lines
     prefix = key[0].split("[")[0]
     outs = {}
-    for nb, wg in ((1, 256), (1, 512)):
+    for nb, wg in tiles:
         forced = tmp_path / f"tune_{wg}.json"
         forced.write_text('{"version": 2, "entries": {\n' + prefix + f"[{n}, 1, 0, {nb}, {wg}]\n" + "}}\n")
         monkeypatch.setenv("GFFTX_TUNE_CACHE", str(forced))
-        print(f"case single-pass tile nb={nb} wg={wg}", flush=True)
+        print(f"case single-pass tile n={n} nb={nb} wg={wg}", flush=True)
         with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.IN_PLACE, batch=batch) as p:
             desc = json.loads(p.describe())
             assert desc["tuned"] == "cache" and f"_nb{nb}_wg{wg}_" in desc["passes"][0]["kernel"], desc
@@ -305,7 +307,7 @@ def test_single_pass_tiles_tuned_and_cached(torch_cuda, tmp_path, monkeypatch):
             outs[wg] = host(d)
         for b in range(batch):
             assert rel_l2(outs[wg][b], ref[b]) <= 2e-7, (wg, b)
-    assert rel_l2(outs[256], outs[512]) <= 2e-7
+    assert rel_l2(outs[tiles[0][1]], outs[tiles[1][1]]) <= 2e-7
 
 
 def test_out_of_place_alias_rejected(torch_cuda):
