@@ -168,15 +168,15 @@ void split_choice(const std::vector<int>& c, std::vector<int>& lens, std::vector
     }
 }
 
-// XCD-aware tile order for a pass tile (GFFTX_XCD overrides): on when the tile's rows are
-// narrower than a 128-byte line, so that the neighbouring tiles that share each line run on
-// one XCD and meet in its L2.  Tiles with whole-line rows share no lines; they load their
+// XCD-aware tile order for a pass tile (GFFTX_XCD overrides): on when the tile's rows are not
+// whole 128-byte lines, so that the neighbouring tiles that share the lines at their ends run
+// on one XCD and meet in its L2.  Tiles with whole-line rows share no lines; they load their
 // inputs non-temporally and run fastest in dispatch order.
 int xcd_remap(const TileEntry* k, int prec)
 {
     const int forced = env_int("GFFTX_XCD", -1);
     if (forced >= 0) return forced ? 1 : 0;
-    return (size_t)k->nb * elem_bytes(prec) < 128 ? 1 : 0;
+    return (size_t)k->nb * elem_bytes(prec) % 128 != 0 ? 1 : 0;
 }
 
 // Whether n runs natively: one compiled single-pass tile, or a split into compiled pass

@@ -183,6 +183,38 @@ int ragged_launch_all(gfftx_ragged_s* r, const RaggedArgs& a, hipStream_t s)
     return GFFTX_SUCCESS;
 }
 
+// Time of every long class's launches alone (ms, two runs after one warm-up) on side stream
+// 0, in place on a zeroed array of the batch's size (`points` elements).  Leaves `cost`
+// unchanged when the array cannot be allocated or a launch fails.
+void measure_class_costs(gfftx_ragged_s* r, long long points, std::vector<double>& cost)
+{
+    void* tmp = nullptr;
+    const size_t bytes = (size_t)points * elem_bytes(r->prec);
+    if (hipMalloc(&tmp, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
+    }
+    hipStream_t s = r->side[0];
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::vector<double> t(cost.size(), 0.0);
+    bool ok = hipMemsetAsync(tmp, 0, bytes, s) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+              hipEventCreate(&e1) == hipSuccess;
+    for (size_t i = 0; ok && i < r->longs.size(); ++i) {
+        ok = ragged_run_long(r, r->longs[i], tmp, 0, s) == GFFTX_SUCCESS && hipEventRecord(e0, s) == hipSuccess;
+        for (int rep = 0; ok && rep < 2; ++rep) ok = ragged_run_long(r, r->longs[i], tmp, 0, s) == GFFTX_SUCCESS;
+        float ms = 0;
+        ok = ok && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+             hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+        t[i] = ms / 2;
+    }
+    (void)hipStreamSynchronize(s);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(tmp);
+    (void)hipGetLastError();
+    if (ok) cost = t;
+}
+
 int ragged_build(gfftx_ragged_s* r, const uint64_t* lengths, uint64_t count)
 {
     // classes by length, in order of first appearance
@@ -361,21 +393,13 @@ int ragged_build(gfftx_ragged_s* r, const uint64_t* lengths, uint64_t count)
     r->args.classes = (const RaggedClass*)d_classes;
     r->args.counter = (unsigned long long*)d_counter;
     r->args.scratch = (float2*)d_scratch;
+    std::string sched;  // describe(): each long class's cost and side stream
     if (!r->longs.empty()) {
-        // classes to side streams, largest work first onto the least loaded stream
+        // classes to side streams, largest cost first onto the least loaded stream.  The cost
+        // of a class is its measured time on this device (its own launches, on a scratch array
+        // of the batch's size); without a measurement, the points its passes move.
         r->nsides = std::max(1, std::min<int>(env_int("GFFTX_RAGGED_STREAMS", kRaggedSides), kRaggedSides));
         r->nsides = std::min<int>(r->nsides, (int)r->longs.size());
-        std::vector<size_t> order(r->longs.size());
-        for (size_t i = 0; i < order.size(); ++i) order[i] = i;
-        auto work = [&](size_t i) { return (double)r->longs[i].L * (double)r->longs[i].count * std::log2((double)r->longs[i].L); };
-        std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return work(x) > work(y); });
-        std::vector<double> load(r->nsides, 0.0);
-        r->side_of.assign(r->longs.size(), 0);
-        for (size_t i : order) {
-            const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-            r->side_of[i] = k;
-            load[k] += work(i);
-        }
         for (int k = 0; k < r->nsides; ++k) {
             HIP_TRY(hipStreamCreateWithFlags(&r->side[k], hipStreamNonBlocking));
             HIP_TRY(hipEventCreateWithFlags(&r->join_ev[k], hipEventDisableTiming));
@@ -383,6 +407,28 @@ int ragged_build(gfftx_ragged_s* r, const uint64_t* lengths, uint64_t count)
                 e = dev_alloc_list(r->allocs, &r->lscratch[b][k], (size_t)off * elem_bytes(r->prec));
                 if (e) return e;
             }
+        }
+        std::vector<double> cost(r->longs.size());
+        for (size_t i = 0; i < cost.size(); ++i) {
+            const gfftx_ragged_s::LongClass& lc = r->longs[i];
+            cost[i] = (double)lc.L * (double)lc.count * (lc.blue ? 6.0 : (double)std::max<size_t>(1, lc.passes.size()));
+        }
+        if (!(r->flags & GFFTX_FLAG_NO_AUTOTUNE) && env_int("GFFTX_AUTOTUNE", 1) != 0) measure_class_costs(r, off, cost);
+        std::vector<size_t> order(r->longs.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+        std::vector<double> load(r->nsides, 0.0);
+        r->side_of.assign(r->longs.size(), 0);
+        for (size_t i : order) {
+            const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            r->side_of[i] = k;
+            load[k] += cost[i];
+        }
+        for (size_t i = 0; i < r->longs.size(); ++i) {
+            char cb[96];
+            snprintf(cb, sizeof(cb), "%s{\"L\":%lld,\"cost\":%.6g,\"side\":%d}", i ? "," : "", r->longs[i].L, cost[i],
+                     r->side_of[i]);
+            sched += cb;
         }
         HIP_TRY(hipEventCreateWithFlags(&r->fork_ev, hipEventDisableTiming));
     }
@@ -395,7 +441,7 @@ int ragged_build(gfftx_ragged_s* r, const uint64_t* lengths, uint64_t count)
     r->grid1 = std::max(1, std::min<int>((int)items1.size(), cus * per_cu));
     r->desc += "],\"items\":[" + std::to_string(items.size()) + "," + std::to_string(items1.size()) +
                "],\"grid\":[" + std::to_string(r->grid) + "," + std::to_string(r->grid1) +
-               "],\"scratch_points\":" + std::to_string(scratch) + "}";
+               "],\"scratch_points\":" + std::to_string(scratch) + ",\"long_classes\":[" + sched + "]}";
     return GFFTX_SUCCESS;
 }
 

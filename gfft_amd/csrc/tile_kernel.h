@@ -184,9 +184,10 @@ template <class C> struct FirstIn {
 // The input of a pass is dead once read (a scratch array, or the caller's input that the
 // transform consumes), while its output is what the next pass reads.  Loading the input
 // non-temporally lets the caches (L2 and the 256 MB Infinity Cache) keep the output instead:
-// a 2^24 FP32 round trip runs about 30% faster.  Used where the tile's loads cover whole
+// a 2^24 FP32 round trip runs about 30% faster.  Used where the tile's rows are whole
 // 128-byte lines (T mode: NB adjacent sequences per row; E mode: NB whole sequences), so
-// that no other tile needs a line this one evicts early.
+// that no other tile needs a line this one evicts early; a row of, say, 21 sequences
+// (168 bytes) shares its end lines with the neighbouring tiles.
 template <class C, bool IDLE> __device__ __forceinline__ void load_first(const Ctx<C>& x, FirstIn<C>& f)
 {
     using V = typename C::V;
@@ -194,7 +195,7 @@ template <class C, bool IDLE> __device__ __forceinline__ void load_first(const C
     constexpr int B = C::nbf(0);
     constexpr int LR = C::L / R;
     constexpr bool GUARD = (B * C::TPT > LR);
-    constexpr bool NT = (C::LOADT ? C::NB : C::NB * C::L) * (int)sizeof(V) >= 128;
+    constexpr bool NT = (C::LOADT ? C::NB : C::NB * C::L) * (int)sizeof(V) % 128 == 0;
 #pragma unroll
     for (int b = 0; b < B; ++b) {
         const int j = x.jt + b * C::TPT;
