@@ -21,7 +21,8 @@ template <class C> static hipError_t launch_tile(const TileArgs& a, dim3 grid, h
      GFFTX_CFG(TYPE, L, NB, WG, LT, ST, __VA_ARGS__)::R,                                      \
      (int)(GFFTX_CFG(TYPE, L, NB, WG, LT, ST, __VA_ARGS__)::LDS_ELEMS * sizeof(V2<TYPE>)),    \
      &launch_tile<GFFTX_CFG(TYPE, L, NB, WG, LT, ST, __VA_ARGS__)>,                           \
-     "tile_" #TYPE "_L" #L "_nb" #NB "_wg" #WG "_" #LT #ST}
+     "tile_" #TYPE "_L" #L "_nb" #NB "_wg" #WG "_" #LT #ST,                                    \
+     false}
 
 // T->T pass tiles with the R2C split fused in (SplitCfg), one per tile shape of the last
 // passes of real forward transforms; NB must be even.
