@@ -208,6 +208,9 @@ TileEntry kTiles[] = {
     GFFTX_BATCH_POW2(PREC_F64, double),
     GFFTX_BATCH_MIXED(PREC_F64, double),
     GFFTX_PASS_SET(PREC_F64, double, 2, 512, 4, 256, 4, 384),
+    // 1024 points with eight per thread in the first two stages (512 threads): the
+    // autotuner weighs it against the radix-16 tile for batched FP64 transforms
+    GFFTX_TILE(PREC_F64, double, 1024, 4, 512, false, false, 8, 8, 16),
     GFFTX_TILE(PREC_F64, double, 256, 8, 128, true, false, 16, 16),
     GFFTX_TILE(PREC_F64, double, 256, 8, 128, true, true, 16, 16),
     GFFTX_TILE(PREC_F64, double, 4096, 1, 256, true, false, 16, 16, 16),

@@ -271,21 +271,22 @@ def test_autotuner_measures_then_caches(torch_cuda, tmp_path, monkeypatch):
         assert "tuned" not in json.loads(p.describe())
 
 
-@pytest.mark.parametrize("n,batch,tiles", [
-    (4096, 64, ((1, 256), (1, 512))), (2048, 40, ((2, 256), (2, 512))), (5120, 24, ((1, 320), (1, 640))),
-    (6144, 24, ((1, 384), (1, 768))), (8192, 16, ((1, 512), (1, 1024)))])
-def test_single_pass_tiles_tuned_and_cached(torch_cuda, tmp_path, monkeypatch, n, batch, tiles):
+@pytest.mark.parametrize("dtype,n,batch,tiles", [
+    (np.complex64, 4096, 64, ((1, 256), (1, 512))), (np.complex64, 2048, 40, ((2, 256), (2, 512))),
+    (np.complex64, 5120, 24, ((1, 320), (1, 640))), (np.complex64, 6144, 24, ((1, 384), (1, 768))),
+    (np.complex64, 8192, 16, ((1, 512), (1, 1024))), (np.complex128, 1024, 96, ((4, 256), (4, 512)))])
+def test_single_pass_tiles_tuned_and_cached(torch_cuda, tmp_path, monkeypatch, dtype, n, batch, tiles):
     """A batched single-pass plan whose length has several compiled tiles (radix-16 stages,
     and radix-8 stages with twice the threads) times them and caches the choice as
     [n, 1, 0, nb, wg].  Each tile, forced through its own cache file, matches the oracle, and
     the two agree."""
     import json
 
-    x = np.stack([inputs.complex_normal(n, 300 + b) for b in range(batch)]).astype(np.complex64)
+    x = np.stack([inputs.complex_normal(n, 300 + b) for b in range(batch)]).astype(dtype)
     ref = np.stack([oracle_fwd(r) for r in x])
     cache = tmp_path / "tune.json"
     monkeypatch.setenv("GFFTX_TUNE_CACHE", str(cache))
-    with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.IN_PLACE, batch=batch) as p:
+    with api.Plan(n, VT[dtype], api.DFT, api.IN_PLACE, batch=batch) as p:
         desc = json.loads(p.describe())
     assert isinstance(desc["tuned"], list) and len(desc["tuned"]) >= 2, desc
     assert all(t["split"].startswith("single/") for t in desc["tuned"])
@@ -299,15 +300,15 @@ def test_single_pass_tiles_tuned_and_cached(torch_cuda, tmp_path, monkeypatch, n
         forced.write_text('{"version": 2, "entries": {\n' + prefix + f"[{n}, 1, 0, {nb}, {wg}]\n" + "}}\n")
         monkeypatch.setenv("GFFTX_TUNE_CACHE", str(forced))
         print(f"case single-pass tile n={n} nb={nb} wg={wg}", flush=True)
-        with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.IN_PLACE, batch=batch) as p:
+        with api.Plan(n, VT[dtype], api.DFT, api.IN_PLACE, batch=batch) as p:
             desc = json.loads(p.describe())
             assert desc["tuned"] == "cache" and f"_nb{nb}_wg{wg}_" in desc["passes"][0]["kernel"], desc
             d = dev(torch_cuda, x)
             p.fft(d)
             outs[wg] = host(d)
         for b in range(batch):
-            assert rel_l2(outs[wg][b], ref[b]) <= 2e-7, (wg, b)
-    assert rel_l2(outs[tiles[0][1]], outs[tiles[1][1]]) <= 2e-7
+            assert rel_l2(outs[wg][b], ref[b]) <= TOL[dtype], (wg, b)
+    assert rel_l2(outs[tiles[0][1]], outs[tiles[1][1]]) <= TOL[dtype]
 
 
 def test_out_of_place_alias_rejected(torch_cuda):
