@@ -211,6 +211,8 @@ TileEntry kTiles[] = {
     // 1024 points with eight per thread in the first two stages (512 threads): the
     // autotuner weighs it against the radix-16 tile for batched FP64 transforms
     GFFTX_TILE(PREC_F64, double, 1024, 4, 512, false, false, 8, 8, 16),
+    GFFTX_TILE(PREC_F64, double, 256, 16, 512, true, false, 8, 8, 4),
+    GFFTX_TILE(PREC_F64, double, 256, 16, 512, true, true, 8, 8, 4),
     GFFTX_TILE(PREC_F64, double, 256, 8, 128, true, false, 16, 16),
     GFFTX_TILE(PREC_F64, double, 256, 8, 128, true, true, 16, 16),
     GFFTX_TILE(PREC_F64, double, 4096, 1, 256, true, false, 16, 16, 16),

@@ -311,6 +311,29 @@ def test_single_pass_tiles_tuned_and_cached(torch_cuda, tmp_path, monkeypatch, d
     assert rel_l2(outs[tiles[0][1]], outs[tiles[1][1]]) <= TOL[dtype]
 
 
+@pytest.mark.parametrize("dtype,n,passes,tnb,twg", [
+    (np.complex128, 2 ** 16, "256,256", 16, 512), (np.complex128, 2 ** 24, "256,256,256", 16, 512)])
+def test_forced_pass_tiles(torch_cuda, monkeypatch, dtype, n, passes, tnb, twg):
+    """Pass tiles forced through GFFTX_TNB / GFFTX_TWG (here the FP64 256-point tile with
+    radix 8x8x4 stages and 512 threads): forward against the oracle (numpy's FFT at 2^24),
+    and the inverse back to the input."""
+    monkeypatch.setenv("GFFTX_PASSES", passes)
+    monkeypatch.setenv("GFFTX_TNB", str(tnb))
+    monkeypatch.setenv("GFFTX_TWG", str(twg))
+    x = inputs.complex_normal(n, 500 + n % 997).astype(dtype)
+    ref = oracle_fwd(x) if n <= 2 ** 16 else np.fft.fft(x.astype(np.complex128))
+    print(f"case forced pass tiles n={n} nb={tnb} wg={twg}", flush=True)
+    with api.Plan(n, VT[dtype], api.DFT, api.IN_PLACE) as pf, api.Plan(n, VT[dtype], api.IDFT, api.IN_PLACE) as pi:
+        import json
+
+        assert all(f"_nb{tnb}_wg{twg}_" in ps["kernel"] for ps in json.loads(pf.describe())["passes"])
+        d = dev(torch_cuda, x)
+        pf.fft(d)
+        assert rel_l2(host(d), ref) <= TOL[dtype]
+        pi.fft(d)
+        assert rel_l2(host(d), x) <= 2 * TOL[dtype]
+
+
 def test_out_of_place_alias_rejected(torch_cuda):
     with api.Plan(256, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE) as p:
         d = torch_cuda.zeros(256, dtype=torch_cuda.complex64, device="cuda")
