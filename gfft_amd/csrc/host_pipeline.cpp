@@ -5,11 +5,13 @@
 namespace gfftx {
 namespace detail {
 
-// Copy between pageable and pinned host memory with up to 8 threads (chunks of 1 MiB or more).
+// Copy between pageable and pinned host memory with up to GFFTX_HOST_THREADS threads
+// (default 8; chunks of 1 MiB or more).
 void host_copy(void* dst, const void* src, size_t len)
 {
     unsigned t = std::thread::hardware_concurrency();
-    t = std::min<unsigned>(std::max(t, 1u), 8u);
+    const unsigned cap = (unsigned)std::max(1, std::min(64, env_int("GFFTX_HOST_THREADS", 8)));
+    t = std::min<unsigned>(std::max(t, 1u), cap);
     t = (unsigned)std::min<size_t>(t, len >> 20);
     if (t <= 1) {
         memcpy(dst, src, len);
