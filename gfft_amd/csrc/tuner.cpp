@@ -196,10 +196,10 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
         std::string timings = ",\"tuned\":[";
         for (size_t i = 0; i < cands.size(); ++i) {
             gfftx_plan cp = nullptr;
-            // the candidates are the complex transforms of the plan (a real plan's bins
-            // layout only widens the buffers, which `bytes` already covers)
-            if (create_plan_impl(&cp, n, batch, vtype, GFFTX_DFT, GFFTX_OUT_OF_PLACE, device, 0, cands[i], "") !=
-                GFFTX_SUCCESS)
+            // the candidates are the plan's own transform (a real forward plan times its last
+            // pass with the R2C split fused in), out of place; `bytes` covers the bins layout
+            if (create_plan_impl(&cp, n, batch, vtype, ttype, GFFTX_OUT_OF_PLACE, device, flags & GFFTX_FLAG_R2C_BINS,
+                                 cands[i], "") != GFFTX_SUCCESS)
                 continue;
             float ms = 0;
             const int trc = time_plan(cp, a, b, s, 5, &ms);
