@@ -196,13 +196,6 @@ TileEntry kTiles[] = {
     GFFTX_TILE(PREC_F32, float, 1024, 4, 512, true, false, 8, 8, 8, 2),
     GFFTX_TILE(PREC_F32, float, 1024, 4, 512, true, true, 8, 8, 8, 2),
     GFFTX_SPLIT_TILE(PREC_F32, float, 1024, 4, 512, 8, 8, 8, 2),
-    // 512- and 2048-point passes: two-pass splits of 2^19 .. 2^23 (2048 x 1024 and the like)
-    GFFTX_TILE(PREC_F32, float, 512, 16, 512, true, false, 16, 16, 2),
-    GFFTX_TILE(PREC_F32, float, 512, 16, 512, true, true, 16, 16, 2),
-    GFFTX_SPLIT_TILE(PREC_F32, float, 512, 16, 512, 16, 16, 2),
-    GFFTX_TILE(PREC_F32, float, 2048, 4, 512, true, false, 16, 16, 8),
-    GFFTX_TILE(PREC_F32, float, 2048, 4, 512, true, true, 16, 16, 8),
-    GFFTX_SPLIT_TILE(PREC_F32, float, 2048, 4, 512, 16, 16, 8),
     GFFTX_TILE(PREC_F32, float, 16, 64, 64, true, true, 16),
     GFFTX_TILE(PREC_F32, float, 64, 64, 256, true, true, 16, 4),
     GFFTX_TILE(PREC_F32, float, 16, 64, 64, true, false, 16),

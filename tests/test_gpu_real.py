@@ -174,23 +174,19 @@ def test_narrow_pass_tiles(torch_cuda, monkeypatch, tnb, passes):
 
 @pytest.mark.parametrize("n,passes,tnb,twg", [
     (1536 * 1536, "1536,1536", 4, 768), (1024 * 1024, "1024,1024", 8, 1024),
-    (1024 * 1024, "1024,1024", 4, 512), (2 ** 21, "2048,1024", 4, 512), (2 ** 18, "512,512", 16, 512),
-    (2 ** 21, "2,2048,512", 0, 0)])
+    (1024 * 1024, "1024,1024", 4, 512)])
 def test_radix8_pass_tiles(torch_cuda, monkeypatch, n, passes, tnb, twg):
-    """Pass tiles forced through GFFTX_PASSES, GFFTX_TNB and GFFTX_TWG (eight points per
-    thread with radix-8 stages; the 512- and 2048-point passes): the real forward transform
-    with the fused split, and the complex forward and inverse, against numpy."""
+    """Pass tiles with eight points per thread (radix-8 stages), forced through GFFTX_TNB and
+    GFFTX_TWG: the real forward transform with the fused split, and the complex forward and
+    inverse, against numpy."""
     monkeypatch.setenv("GFFTX_PASSES", passes)
-    if tnb:
-        monkeypatch.setenv("GFFTX_TNB", str(tnb))
-        monkeypatch.setenv("GFFTX_TWG", str(twg))
+    monkeypatch.setenv("GFFTX_TNB", str(tnb))
+    monkeypatch.setenv("GFFTX_TWG", str(twg))
     z = inputs.complex_normal(n, 80 + tnb).astype(np.complex64)
-    print(f"case pass tiles n={n} passes={passes} nb={tnb} wg={twg}", flush=True)
+    print(f"case radix-8 tiles n={n} nb={tnb} wg={twg}", flush=True)
     with api.Plan(n, api.FLOAT, api.RDFT, api.OUT_OF_PLACE) as p:
         desc = json.loads(p.describe())
-        assert [ps["L"] for ps in desc["passes"]] == [int(v) for v in passes.split(",")], desc
-        if tnb:
-            assert all(f"_nb{tnb}_wg{twg}_" in ps["kernel"] for ps in desc["passes"]), desc
+        assert all(f"_nb{tnb}_wg{twg}_" in ps["kernel"] for ps in desc["passes"]), desc
         assert desc["passes"][-1]["kernel"].endswith("_split"), desc
         s = dev(torch_cuda, z)
         o = torch_cuda.empty_like(s)
