@@ -184,13 +184,15 @@ int ragged_launch_all(gfftx_ragged_s* r, const RaggedArgs& a, hipStream_t s)
 }
 
 // Time of every long class's launches alone (ms, two runs after one warm-up) on side stream
-// 0, in place on a zeroed array of the batch's size (`points` elements).  Leaves `cost`
+// 0, in place on a zeroed array of the batch's size (`points` elements): side stream 1's
+// first scratch array when there is one, else a temporary allocation.  Leaves `cost`
 // unchanged when the array cannot be allocated or a launch fails.
 void measure_class_costs(gfftx_ragged_s* r, long long points, std::vector<double>& cost)
 {
-    void* tmp = nullptr;
+    const bool borrowed = r->nsides >= 2 && r->lscratch[0][1] != nullptr;
+    void* tmp = borrowed ? r->lscratch[0][1] : nullptr;
     const size_t bytes = (size_t)points * elem_bytes(r->prec);
-    if (hipMalloc(&tmp, bytes) != hipSuccess) {
+    if (!borrowed && hipMalloc(&tmp, bytes) != hipSuccess) {
         (void)hipGetLastError();
         return;
     }
@@ -210,7 +212,7 @@ void measure_class_costs(gfftx_ragged_s* r, long long points, std::vector<double
     (void)hipStreamSynchronize(s);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    (void)hipFree(tmp);
+    if (!borrowed) (void)hipFree(tmp);
     (void)hipGetLastError();
     if (ok) cost = t;
 }

@@ -182,7 +182,12 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
             }
         if (cands.size() < 2) return GFFTX_SUCCESS;
         DeviceGuard g(device);
-        const size_t bytes = (size_t)p->dist * batch * elem_bytes(p->prec);
+        // the candidates run on at most 1 GiB of data (a larger batch is timed on a prefix of
+        // its items), so tuning never doubles a large plan's memory; the prefix still exceeds
+        // the 256 MB Infinity Cache, so the timings see HBM as the plan will
+        const size_t item_bytes = (size_t)p->dist * elem_bytes(p->prec);
+        const uint64_t tbatch = std::min<uint64_t>(batch, std::max<uint64_t>(1, (1ull << 30) / item_bytes));
+        const size_t bytes = item_bytes * tbatch;
         void *a = nullptr, *b = nullptr;
         if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
             (void)hipGetLastError();
@@ -198,7 +203,7 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
             gfftx_plan cp = nullptr;
             // the candidates are the plan's own transform (a real forward plan times its last
             // pass with the R2C split fused in), out of place; `bytes` covers the bins layout
-            if (create_plan_impl(&cp, n, batch, vtype, ttype, GFFTX_OUT_OF_PLACE, device, flags & GFFTX_FLAG_R2C_BINS,
+            if (create_plan_impl(&cp, n, tbatch, vtype, ttype, GFFTX_OUT_OF_PLACE, device, flags & GFFTX_FLAG_R2C_BINS,
                                  cands[i], "") != GFFTX_SUCCESS)
                 continue;
             float ms = 0;
@@ -275,7 +280,10 @@ bool tune_length_class(int device, int prec, long long L, long long count, bool 
         }
         for (const std::vector<int>& sp : splits) cands.push_back(sp);
         DeviceGuard g(device);
-        const size_t bytes = (size_t)L * (size_t)count * elem_bytes(prec);
+        // at most 1 GiB of data, like plan tuning
+        const uint64_t tcount = std::min<uint64_t>((uint64_t)count,
+                                                   std::max<uint64_t>(1, (1ull << 30) / ((size_t)L * elem_bytes(prec))));
+        const size_t bytes = (size_t)L * tcount * elem_bytes(prec);
         void *a = nullptr, *b = nullptr;
         if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
             (void)hipGetLastError();
@@ -290,7 +298,7 @@ bool tune_length_class(int device, int prec, long long L, long long count, bool 
         bool found = false;
         for (const std::vector<int>& c : cands) {
             gfftx_plan cp = nullptr;
-            if (create_plan_impl(&cp, (uint64_t)L, (uint64_t)count, cvt, inverse ? GFFTX_IDFT : GFFTX_DFT,
+            if (create_plan_impl(&cp, (uint64_t)L, tcount, cvt, inverse ? GFFTX_IDFT : GFFTX_DFT,
                                  GFFTX_OUT_OF_PLACE, device, GFFTX_FLAG_NO_AUTOTUNE, c, "") != GFFTX_SUCCESS)
                 continue;
             float ms = 0;
