@@ -199,20 +199,23 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
         (void)hipMemsetAsync(a, 0, bytes, s);
         float best = 1e30f;
         std::string timings = ",\"tuned\":[";
-        for (size_t i = 0; i < cands.size(); ++i) {
+        size_t ntimed = 0;
+        std::vector<float> cms(cands.size(), -1.f);  // time of each candidate (-1: failed)
+        // time one candidate, record it in `timings`, keep it if it is the fastest so far
+        auto try_cand = [&](const std::vector<int>& c) -> float {
             gfftx_plan cp = nullptr;
             // the candidates are the plan's own transform (a real forward plan times its last
             // pass with the R2C split fused in), out of place; `bytes` covers the bins layout
             if (create_plan_impl(&cp, n, tbatch, vtype, ttype, GFFTX_OUT_OF_PLACE, device, flags & GFFTX_FLAG_R2C_BINS,
-                                 cands[i], "") != GFFTX_SUCCESS)
-                continue;
+                                 c, "") != GFFTX_SUCCESS)
+                return -1.f;
             float ms = 0;
             const int trc = time_plan(cp, a, b, s, 5, &ms);
             gfftx_plan_destroy(cp);
-            if (trc) continue;
+            if (trc) return -1.f;
             std::vector<int> sl;
             std::vector<std::pair<int, int>> tl;
-            split_choice(cands[i], sl, tl);
+            split_choice(c, sl, tl);
             std::string lens;
             if (single_pass) {
                 lens = "single/" + std::to_string(tl[0].first) + "-" + std::to_string(tl[0].second);
@@ -223,12 +226,43 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
                 }
             }
             char buf[160];
-            snprintf(buf, sizeof(buf), "%s{\"split\":\"%s\",\"ms\":%.4f}", i ? "," : "", lens.c_str(), ms);
+            snprintf(buf, sizeof(buf), "%s{\"split\":\"%s\",\"ms\":%.4f}", ntimed++ ? "," : "", lens.c_str(), ms);
             timings += buf;
             if (ms < best) {
                 best = ms;
-                chosen = cands[i];
+                chosen = c;
             }
+            return ms;
+        };
+        for (size_t i = 0; i < cands.size(); ++i) cms[i] = try_cand(cands[i]);
+        // The candidates vary one pass position at a time around the default tiles.  Combine,
+        // for the winning split, the fastest variant seen at each position, and time that too.
+        if (!single_pass && !chosen.empty()) {
+            std::vector<int> wl;
+            std::vector<std::pair<int, int>> wt;
+            split_choice(chosen, wl, wt);
+            std::vector<std::pair<int, int>> bt(wl.size(), {0, 0});
+            std::vector<float> bms(wl.size(), 1e30f);
+            for (size_t i = 0; i < cands.size(); ++i) {
+                if (cms[i] < 0) continue;
+                std::vector<int> cl;
+                std::vector<std::pair<int, int>> ct;
+                split_choice(cands[i], cl, ct);
+                if (cl != wl) continue;
+                for (size_t j = 0; j < ct.size() && j < wl.size(); ++j)
+                    if (ct[j].first && cms[i] < bms[j]) {
+                        bms[j] = cms[i];
+                        bt[j] = ct[j];
+                    }
+            }
+            std::vector<int> combo = wl;
+            combo.push_back(0);
+            for (const auto& t : bt) {
+                combo.push_back(t.first);
+                combo.push_back(t.second);
+            }
+            const bool varied = std::any_of(bt.begin(), bt.end(), [](const std::pair<int, int>& t) { return t.first != 0; });
+            if (varied && combo != chosen && std::find(cands.begin(), cands.end(), combo) == cands.end()) try_cand(combo);
         }
         timings += "]";
         (void)hipStreamSynchronize(s);
