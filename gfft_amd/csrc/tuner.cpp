@@ -132,7 +132,9 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
     if ((p->passes.size() < 2 && !single_pass) || (flags & GFFTX_FLAG_NO_AUTOTUNE) ||
         env_int("GFFTX_AUTOTUNE", 1) == 0 || getenv("GFFTX_PASSES"))
         return GFFTX_SUCCESS;
-    const std::string key = tune_key(device, p->prec, n, batch);
+    // a real forward plan is timed with the R2C split fused into its last pass, so its choice
+    // is kept apart from the complex transforms of the same length
+    const std::string key = tune_key(device, p->prec, n, batch) + (ttype == GFFTX_RDFT ? "|rdft" : "");
     std::vector<int> chosen;
     std::string note;
     {
