@@ -207,12 +207,14 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
         auto try_cand = [&](const std::vector<int>& c) -> float {
             gfftx_plan cp = nullptr;
             // the candidates are the plan's own transform (a real forward plan times its last
-            // pass with the R2C split fused in), out of place; `bytes` covers the bins layout
-            if (create_plan_impl(&cp, n, tbatch, vtype, ttype, GFFTX_OUT_OF_PLACE, device, flags & GFFTX_FLAG_R2C_BINS,
-                                 c, "") != GFFTX_SUCCESS)
+            // pass with the R2C split fused in) with its own placement: an in-place plan runs
+            // on one buffer, which the caches treat differently from two; `bytes` covers the
+            // bins layout
+            if (create_plan_impl(&cp, n, tbatch, vtype, ttype, place, device, flags & GFFTX_FLAG_R2C_BINS, c, "") !=
+                GFFTX_SUCCESS)
                 return -1.f;
             float ms = 0;
-            const int trc = time_plan(cp, a, b, s, 5, &ms);
+            const int trc = time_plan(cp, a, place == GFFTX_IN_PLACE ? a : b, s, 5, &ms);
             gfftx_plan_destroy(cp);
             if (trc) return -1.f;
             std::vector<int> sl;
@@ -320,10 +322,9 @@ bool tune_length_class(int device, int prec, long long L, long long count, bool 
         const uint64_t tcount = std::min<uint64_t>((uint64_t)count,
                                                    std::max<uint64_t>(1, (1ull << 30) / ((size_t)L * elem_bytes(prec))));
         const size_t bytes = (size_t)L * tcount * elem_bytes(prec);
-        void *a = nullptr, *b = nullptr;
-        if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+        void* a = nullptr;
+        if (hipMalloc(&a, bytes) != hipSuccess) {
             (void)hipGetLastError();
-            if (a) (void)hipFree(a);
             return false;
         }
         hipStream_t s = nullptr;
@@ -334,11 +335,12 @@ bool tune_length_class(int device, int prec, long long L, long long count, bool 
         bool found = false;
         for (const std::vector<int>& c : cands) {
             gfftx_plan cp = nullptr;
-            if (create_plan_impl(&cp, (uint64_t)L, tcount, cvt, inverse ? GFFTX_IDFT : GFFTX_DFT,
-                                 GFFTX_OUT_OF_PLACE, device, GFFTX_FLAG_NO_AUTOTUNE, c, "") != GFFTX_SUCCESS)
+            // in place, as the classes of a ragged batch run
+            if (create_plan_impl(&cp, (uint64_t)L, tcount, cvt, inverse ? GFFTX_IDFT : GFFTX_DFT, GFFTX_IN_PLACE,
+                                 device, GFFTX_FLAG_NO_AUTOTUNE, c, "") != GFFTX_SUCCESS)
                 continue;
             float ms = 0;
-            const int trc = time_plan(cp, a, b, s, 5, &ms);
+            const int trc = time_plan(cp, a, a, s, 5, &ms);
             gfftx_plan_destroy(cp);
             if (trc) continue;
             if (ms < best) {
@@ -350,7 +352,6 @@ bool tune_length_class(int device, int prec, long long L, long long count, bool 
         (void)hipStreamSynchronize(s);
         (void)hipStreamDestroy(s);
         (void)hipFree(a);
-        (void)hipFree(b);
         (void)hipGetLastError();
         if (!found) return false;
         std::lock_guard<std::mutex> lk(g_tune_mu);
