@@ -129,17 +129,29 @@ def parse():
     return ap.parse_args()
 
 
+def _settle(torch, run, seconds=0.3):
+    """Issues run() until `seconds` of wall time have passed, then waits: the device reaches
+    its sustained clocks before a measurement (after a host-side pause, the first tens of
+    milliseconds of work measured 5-10% slower)."""
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(4):
+            run()
+        torch.cuda.synchronize()
+
+
 def round_trip_run(torch, api, n, dtype, steps, warmup, nbuf, seed=24):
-    """Times `steps` forward+inverse round trips (plans replayed as hipGraphs, no per-kernel
-    events inside the timed region), then measures per-kernel durations in a separate loop
-    with GFFTX_FLAG_PROFILE plans of the same split."""
+    """Times `steps` forward+inverse round trips (kernels issued directly on the stream, no
+    per-kernel events inside the timed region), then the same loop with the plans replayed
+    as hipGraphs, then per-kernel durations in a separate loop with GFFTX_FLAG_PROFILE plans
+    of the same split."""
     from gfft_amd import inputs
 
     vt = api.COMPLEX_FLOAT if dtype == np.complex64 else api.COMPLEX_DOUBLE
     x = inputs.complex_normal(n, seed).astype(dtype)
     bufs = [torch.from_numpy(x).to("cuda") for _ in range(nbuf)]
-    pf = api.Plan(n, vt, api.DFT, api.IN_PLACE, flags=api.FLAG_GRAPH)
-    pi = api.Plan(n, vt, api.IDFT, api.IN_PLACE, flags=api.FLAG_GRAPH)
+    pf = api.Plan(n, vt, api.DFT, api.IN_PLACE)
+    pi = api.Plan(n, vt, api.IDFT, api.IN_PLACE)
     stream = torch.cuda.current_stream()
     # one round trip: the identity up to rounding
     pf.fft(bufs[0])
@@ -147,6 +159,15 @@ def round_trip_run(torch, api, n, dtype, steps, warmup, nbuf, seed=24):
     torch.cuda.synchronize()
     rt_err = float(np.linalg.norm(bufs[0].cpu().numpy() - x) / np.linalg.norm(x))
     bufs[0].copy_(torch.from_numpy(x))
+    ctr = [0]
+
+    def one(a, b):
+        buf = bufs[ctr[0] % nbuf]
+        ctr[0] += 1
+        a.fft(buf)
+        b.fft(buf)
+
+    _settle(torch, lambda: one(pf, pi))
     for i in range(max(warmup, nbuf)):
         b = bufs[i % nbuf]
         pf.fft(b)
@@ -166,6 +187,12 @@ def round_trip_run(torch, api, n, dtype, steps, warmup, nbuf, seed=24):
     desc = pf.describe()
     pf.close()
     pi.close()
+    # the same loop with each direction replayed as one captured graph
+    gf = api.Plan(n, vt, api.DFT, api.IN_PLACE, flags=api.FLAG_GRAPH)
+    gi = api.Plan(n, vt, api.IDFT, api.IN_PLACE, flags=api.FLAG_GRAPH)
+    graph_ms = _call_time_us(torch, lambda: one(gf, gi), iters=steps) * 1e-3
+    gf.close()
+    gi.close()
     # per-kernel durations (HIP events on the launch stream around every kernel)
     qf = api.Plan(n, vt, api.DFT, api.IN_PLACE, flags=api.FLAG_PROFILE)
     qi = api.Plan(n, vt, api.IDFT, api.IN_PLACE, flags=api.FLAG_PROFILE)
@@ -185,7 +212,8 @@ def round_trip_run(torch, api, n, dtype, steps, warmup, nbuf, seed=24):
     qi.close()
     del bufs
     torch.cuda.empty_cache()
-    return {"ms": ms, "wall_s": wall, "profile": prof, "rt_err": rt_err, "plan": desc}
+    return {"ms": ms, "wall_s": wall, "profile": prof, "rt_err": rt_err, "plan": desc,
+            "graph_ms_per_step": graph_ms}
 
 
 def _kernel_time_us(plan, run, iters=20, warm=3):
@@ -202,14 +230,12 @@ def _kernel_time_us(plan, run, iters=20, warm=3):
     return [(name, tot / cnt * 1e3) for name, tot, cnt in plan.profile()]
 
 
-def _call_time_us(torch, run, iters=50, warm=5):
+def _call_time_us(torch, run, iters=50):
     """Average time (us) of one call of run(), all of its launches, over `iters` calls issued
-    back to back (a FLAG_GRAPH plan replays one captured graph per call), from events around
-    the whole loop on the calling stream.  Unlike the per-launch event pairs, this leaves out
-    the launch overhead that small kernels would otherwise carry each."""
-    for _ in range(warm):
-        run()
-    torch.cuda.synchronize()
+    back to back, from events around the whole loop on the calling stream.  Unlike the
+    per-launch event pairs, this leaves out the launch overhead that small kernels would
+    otherwise carry each."""
+    _settle(torch, run)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(iters):
@@ -227,24 +253,26 @@ def other_workloads(torch, api):
     # C1: N=2^10 FP64 in place forward, 2^16 copies per call
     n, batch = 1024, 65536
     x = torch.from_numpy(inputs.complex_uniform(n * batch, 17)).cuda()
-    with api.Plan(n, api.DOUBLE, api.DFT, api.IN_PLACE, batch=batch, flags=api.FLAG_PROFILE) as p:
-        ks = _kernel_time_us(p, lambda: p.fft(x))
-    us = sum(t for _, t in ks)
+    with api.Plan(n, api.DOUBLE, api.DFT, api.IN_PLACE, batch=batch) as p:
+        us = _call_time_us(torch, lambda: p.fft(x), iters=30)
+        kernel = [k["kernel"] for k in json.loads(p.describe())["passes"]]
     res["c1_fp64_1024_x65536"] = {"us": round(us, 1), "GFLOPs": round(5 * n * 10 * batch / us / 1e3, 1),
-                                  "GBps_alg": round(2 * n * batch * 16 / us / 1e3, 1)}
+                                  "GBps_alg": round(2 * n * batch * 16 / us / 1e3, 1), "kernels": kernel}
     del x
     # C3: R2C N = 3*2^20 real FP32 -> N/2+1 bins
     nr = 3 * 2 ** 20
     xr, _, _ = inputs.real_sinusoids(nr, 3)
     z = torch.from_numpy(xr.astype(np.float32)).cuda()
     o = torch.zeros(nr // 2 + 1, dtype=torch.complex64, device="cuda")
-    # whole-call time from graph replays; the per-kernel event pairs (which each include a
-    # launch) give the breakdown
+    # whole-call time of calls issued back to back, then of graph replays; the per-kernel
+    # event pairs (which each include a launch) give the breakdown
+    with api.Plan(nr // 2, api.FLOAT, api.RDFT, api.OUT_OF_PLACE, flags=api.FLAG_R2C_BINS) as pd:
+        us = _call_time_us(torch, lambda: pd.fft(z, o), iters=200)
     with api.Plan(nr // 2, api.FLOAT, api.RDFT, api.OUT_OF_PLACE, flags=api.FLAG_R2C_BINS | api.FLAG_GRAPH) as pg:
-        us = _call_time_us(torch, lambda: pg.fft(z, o))
+        us_graph = _call_time_us(torch, lambda: pg.fft(z, o), iters=200)
     with api.Plan(nr // 2, api.FLOAT, api.RDFT, api.OUT_OF_PLACE, flags=api.FLAG_R2C_BINS | api.FLAG_PROFILE) as p:
         ks = _kernel_time_us(p, lambda: p.fft(z, o))
-    res["c3_r2c_3x2p20_f32"] = {"us": round(us, 1),
+    res["c3_r2c_3x2p20_f32"] = {"us": round(us, 1), "graph_us": round(us_graph, 1),
                                 "GFLOPs_2.5NlogN": round(2.5 * nr * np.log2(nr) / us / 1e3, 1),
                                 "GBps_alg": round((4 * nr + 8 * (nr // 2 + 1)) / us / 1e3, 1),
                                 "kernels": [(k, round(t, 1)) for k, t in ks],
@@ -254,19 +282,16 @@ def other_workloads(torch, api):
     lengths = [int(v) for v in inputs.ragged_lengths(4096, 4096)]
     tot = sum(lengths)
     xg = torch.from_numpy(inputs.complex_normal(tot, 4096).astype(np.complex64)).cuda()
-    rp = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT, flags=api.FLAG_PROFILE | api.FLAG_GRAPH)
-    for _ in range(3):
-        rp.fft(xg)
-    torch.cuda.synchronize()
-    rp.profile()
-    for _ in range(20):
-        rp.fft(xg)
-    ms, cnt = rp.profile()
-    us = ms / cnt * 1e3
-    flops = sum(5 * L * np.log2(L) for L in lengths)
-    res["c4_ragged_4096_f32"] = {"us": round(us, 1), "GFLOPs": round(flops / us / 1e3, 1),
-                                 "GBps_alg": round(2 * tot * 8 / us / 1e3, 1), "points": tot}
+    rp = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT)
+    us = _call_time_us(torch, lambda: rp.fft(xg), iters=30)
     rp.close()
+    rp = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT, flags=api.FLAG_GRAPH)
+    us_graph = _call_time_us(torch, lambda: rp.fft(xg), iters=30)
+    rp.close()
+    flops = sum(5 * L * np.log2(L) for L in lengths)
+    res["c4_ragged_4096_f32"] = {"us": round(us, 1), "graph_us": round(us_graph, 1),
+                                 "GFLOPs": round(flops / us / 1e3, 1),
+                                 "GBps_alg": round(2 * tot * 8 / us / 1e3, 1), "points": tot}
     del xg
     # C5: 16384 x 4096 FP32 plain vs split-precision MFMA; accuracy against the FP64 engine
     n, batch = 4096, 16384
@@ -276,11 +301,11 @@ def other_workloads(torch, api):
     c5 = {}
     for label, flags in (("plain", 0), ("mfma", api.FLAG_SPLIT_MFMA)):
         with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE, batch=batch,
-                      flags=flags | api.FLAG_PROFILE) as p:
-            ks = _kernel_time_us(p, lambda: p.fft(x32, out))
+                      flags=flags) as p:
+            us = _call_time_us(torch, lambda: p.fft(x32, out), iters=30)
+            kernel = json.loads(p.describe())["passes"][0]["kernel"]
             rows = out.cpu().numpy()[::64]
-        us = sum(t for _, t in ks)
-        c5[label] = {"kernel": ks[0][0], "us": round(us, 1),
+        c5[label] = {"kernel": kernel, "us": round(us, 1),
                      "GFLOPs": round(5 * n * 12 * batch / us / 1e3, 1),
                      "GBps_alg": round(2 * n * batch * 8 / us / 1e3, 1), "rows": rows}
     del x32, out
@@ -299,12 +324,14 @@ def other_workloads(torch, api):
     n = 1048573  # prime
     xp = torch.from_numpy(inputs.complex_normal(n, 13).astype(np.complex64)).cuda()
     op = torch.empty_like(xp)
+    with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE) as pd:
+        us = _call_time_us(torch, lambda: pd.fft(xp, op), iters=100)
     with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE, flags=api.FLAG_GRAPH) as pg:
-        us = _call_time_us(torch, lambda: pg.fft(xp, op))
+        us_graph = _call_time_us(torch, lambda: pg.fft(xp, op), iters=100)
     with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE, flags=api.FLAG_PROFILE) as p:
         ks = _kernel_time_us(p, lambda: p.fft(xp, op))
         m = json.loads(p.describe())["bluestein"]["m"]
-    res["chirp_z_prime_1048573_f32"] = {"us": round(us, 1), "m": m,
+    res["chirp_z_prime_1048573_f32"] = {"us": round(us, 1), "graph_us": round(us_graph, 1), "m": m,
                                         "GFLOPs_5NlogN": round(5 * n * np.log2(n) / us / 1e3, 1),
                                         "GBps_caller": round(2 * n * 8 / us / 1e3, 1),
                                         "kernels": len(ks),
@@ -436,7 +463,8 @@ def main():
                                f"multi-pass Stockham, {nbuf32} rotating buffers",
                    "n": n, "passes": len(json.loads(r32["plan"])["passes"]),
                    "split": [p_["L"] for p_ in json.loads(r32["plan"])["passes"]],
-                   "launch": "hipGraph replay per direction",
+                   "launch": "kernels issued directly on one stream",
+                   "graph_replay_ms_per_step": round(r32["graph_ms_per_step"], 4),
                    "round_trip_relL2": r32["rt_err"]},
         "alg_GBps": round(2 * 2 * n * 8 / (ms_step * 1e-3) / 1e9, 1),
         "roofline": roofline,

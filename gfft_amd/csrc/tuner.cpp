@@ -88,7 +88,10 @@ std::string tune_key(int device, int prec, uint64_t n, uint64_t batch)
     return dev + "|" + (prec == PREC_F64 ? "f64" : "f32") + "|n=" + std::to_string(n) + "|b=" + std::to_string(batch);
 }
 
-// Median of `reps` timed executions (ms) of a plan on scratch buffers.
+// Time (ms) of one execution of a plan on scratch buffers, as a throughput: three groups of
+// `reps` executions issued back to back between two events, the median of the group means.
+// Back-to-back calls are what a batch loop issues; a call timed alone after a synchronisation
+// also carries the start of an idle device.
 int time_plan(gfftx_plan_s* p, void* a, void* b, hipStream_t s, int reps, float* ms_out)
 {
     for (int i = 0; i < 2; ++i) {
@@ -99,19 +102,20 @@ int time_plan(gfftx_plan_s* p, void* a, void* b, hipStream_t s, int reps, float*
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    for (int i = 0; i < reps; ++i) {
+    for (int g = 0; g < 3; ++g) {
         (void)hipEventRecord(e0, s);
-        int rc = exec_device(p, a, b, s);
+        int rc = GFFTX_SUCCESS;
+        for (int i = 0; i < reps && !rc; ++i) rc = exec_device(p, a, b, s);
         (void)hipEventRecord(e1, s);
+        (void)hipEventSynchronize(e1);
         if (rc) {
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
             return rc;
         }
-        (void)hipEventSynchronize(e1);
         float ms = 0;
         (void)hipEventElapsedTime(&ms, e0, e1);
-        t.push_back(ms);
+        t.push_back(ms / reps);
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
