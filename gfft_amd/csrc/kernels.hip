@@ -1,11 +1,15 @@
 // kernels.hip -- instantiated FFT kernels for gfx950 and their host-visible registry.
+#include <hip/hip_ext.h>
+
 #include "registry.h"
 
 namespace gfftx {
 
-template <class C> static hipError_t launch_tile(const TileArgs& a, dim3 grid, hipStream_t s)
+template <class C>
+static hipError_t launch_tile(const TileArgs& a, dim3 grid, hipStream_t s, hipEvent_t t0, hipEvent_t t1)
 {
-    hipLaunchKernelGGL(tile_kernel<C>, grid, dim3(C::WG), 0, s, a);
+    if (t0) hipExtLaunchKernelGGL(tile_kernel<C>, grid, dim3(C::WG), 0, s, t0, t1, 0, a);
+    else hipLaunchKernelGGL(tile_kernel<C>, grid, dim3(C::WG), 0, s, a);
     return hipGetLastError();
 }
 
@@ -229,9 +233,11 @@ TileEntry kTiles[] = {
 };
 const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
-template <class C> static hipError_t launch_mfma(const MfmaArgs& a, dim3 grid, hipStream_t s)
+template <class C>
+static hipError_t launch_mfma(const MfmaArgs& a, dim3 grid, hipStream_t s, hipEvent_t t0, hipEvent_t t1)
 {
-    hipLaunchKernelGGL(mfma_fft_kernel<C>, grid, dim3(C::WG), 0, s, a);
+    if (t0) hipExtLaunchKernelGGL(mfma_fft_kernel<C>, grid, dim3(C::WG), 0, s, t0, t1, 0, a);
+    else hipLaunchKernelGGL(mfma_fft_kernel<C>, grid, dim3(C::WG), 0, s, a);
     return hipGetLastError();
 }
 

@@ -201,7 +201,10 @@ bool is_device_ptr(const void* ptr)
 }
 
 // Event pair for launch slot `slot` (GFFTX_FLAG_PROFILE); returns false when not profiling.
-bool prof_begin(gfftx_plan_s* p, size_t slot, const char* name, hipStream_t s, std::pair<hipEvent_t, hipEvent_t>* ev)
+// With `attached` the pair goes to the launch itself (hipExtLaunchKernel stamps the kernel's
+// start and end); otherwise the events are recorded on s around the launch.
+bool prof_begin(gfftx_plan_s* p, size_t slot, const char* name, hipStream_t s, std::pair<hipEvent_t, hipEvent_t>* ev,
+                bool attached)
 {
     if (!(p->flags & GFFTX_FLAG_PROFILE) || p->capturing) return false;
     if (p->slots.size() <= slot) p->slots.resize(slot + 1);
@@ -217,13 +220,13 @@ bool prof_begin(gfftx_plan_s* p, size_t slot, const char* name, hipStream_t s, s
     }
     *ev = sl.pool.back();
     sl.pool.pop_back();
-    (void)hipEventRecord(ev->first, s);
+    if (!attached) (void)hipEventRecord(ev->first, s);
     return true;
 }
 
-void prof_end(gfftx_plan_s* p, size_t slot, hipStream_t s, const std::pair<hipEvent_t, hipEvent_t>& ev)
+void prof_end(gfftx_plan_s* p, size_t slot, hipStream_t s, const std::pair<hipEvent_t, hipEvent_t>& ev, bool attached)
 {
-    (void)hipEventRecord(ev.second, s);
+    if (!attached) (void)hipEventRecord(ev.second, s);
     p->slots[slot].pending.push_back(ev);
 }
 
@@ -263,9 +266,9 @@ int run_passes(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, Range 
             ma.scale = p->inverse ? (float)(1.0 / (double)p->n) : 1.0f;
             dim3 grid((unsigned)((ps.ntrans + ps.mfma->nb - 1) / ps.mfma->nb));
             std::pair<hipEvent_t, hipEvent_t> ev;
-            const bool on = prof_begin(prof, slot0 + i, ps.mfma->name, s, &ev);
-            hipError_t e = ps.mfma->launch(ma, grid, s);
-            if (on) prof_end(prof, slot0 + i, s, ev);
+            const bool on = prof_begin(prof, slot0 + i, ps.mfma->name, s, &ev, true);
+            hipError_t e = ps.mfma->launch(ma, grid, s, on ? ev.first : nullptr, on ? ev.second : nullptr);
+            if (on) prof_end(prof, slot0 + i, s, ev, true);
             if (e != hipSuccess) return fail(GFFTX_ERR_LAUNCH, "%s: %s", ps.mfma->name, hipGetErrorString(e));
             src = dst[i];
             continue;
@@ -301,9 +304,9 @@ int run_passes(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, Range 
         }
         dim3 grid(ntiles, ps.items_in_ntrans ? 1u : (unsigned)r.nb);
         std::pair<hipEvent_t, hipEvent_t> ev;
-        const bool on = prof_begin(prof, slot0 + i, ps.k->name, s, &ev);
-        hipError_t e = ps.k->launch(a, grid, s);
-        if (on) prof_end(prof, slot0 + i, s, ev);
+        const bool on = prof_begin(prof, slot0 + i, ps.k->name, s, &ev, true);
+        hipError_t e = ps.k->launch(a, grid, s, on ? ev.first : nullptr, on ? ev.second : nullptr);
+        if (on) prof_end(prof, slot0 + i, s, ev, true);
         if (e != hipSuccess) return fail(GFFTX_ERR_LAUNCH, "%s: %s", ps.k->name, hipGetErrorString(e));
         src = dst[i];
     }
@@ -320,9 +323,9 @@ size_t c2c_slots(const gfftx_plan_s* p)
 int blue_launch(gfftx_plan_s* p, int which, const BluesteinArgs& a, size_t slot, const char* name, hipStream_t s)
 {
     std::pair<hipEvent_t, hipEvent_t> ev;
-    const bool on = prof_begin(p, slot, name, s, &ev);
+    const bool on = prof_begin(p, slot, name, s, &ev, false);
     hipError_t e = launch_bluestein(p->prec, which, a, s);
-    if (on) prof_end(p, slot, s, ev);
+    if (on) prof_end(p, slot, s, ev, false);
     if (e != hipSuccess) return fail(GFFTX_ERR_LAUNCH, "%s: %s", name, hipGetErrorString(e));
     return GFFTX_SUCCESS;
 }
@@ -381,9 +384,9 @@ int run_split(gfftx_plan_s* p, void* data, hipStream_t s, Range r)
     a.bins = (p->flags & GFFTX_FLAG_R2C_BINS) ? 1 : 0;
     std::pair<hipEvent_t, hipEvent_t> ev;
     const size_t slot = c2c_slots(p);
-    const bool prof = prof_begin(p, slot, "split_kernel", s, &ev);
+    const bool prof = prof_begin(p, slot, "split_kernel", s, &ev, false);
     hipError_t e = launch_split(p->prec, a, s);
-    if (prof) prof_end(p, slot, s, ev);
+    if (prof) prof_end(p, slot, s, ev, false);
     if (e != hipSuccess) return fail(GFFTX_ERR_LAUNCH, "split: %s", hipGetErrorString(e));
     return GFFTX_SUCCESS;
 }

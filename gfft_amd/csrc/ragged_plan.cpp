@@ -153,7 +153,7 @@ int ragged_run_long(gfftx_ragged_s* r, const gfftx_ragged_s::LongClass& lc, void
             a.xcd = ps.xcd;
             grid = dim3((unsigned)((ps.ntrans + ps.k->nb - 1) / ps.k->nb), (unsigned)lc.count);
         }
-        hipError_t le = ps.k->launch(a, grid, s);
+        hipError_t le = ps.k->launch(a, grid, s, nullptr, nullptr);
         if (le != hipSuccess) return fail(GFFTX_ERR_LAUNCH, "%s: %s", ps.k->name, hipGetErrorString(le));
     }
     return GFFTX_SUCCESS;

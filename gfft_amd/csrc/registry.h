@@ -9,7 +9,10 @@ namespace gfftx {
 
 enum Prec { PREC_F64 = 0, PREC_F32 = 1 };
 
-typedef hipError_t (*TileLaunch)(const TileArgs&, dim3, hipStream_t);
+// Launchers take an optional event pair (both null or both created): when given, the
+// launch goes through hipExtLaunchKernel, which stamps the events with the kernel's own
+// start and end instead of recording them as separate stream commands.
+typedef hipError_t (*TileLaunch)(const TileArgs&, dim3, hipStream_t, hipEvent_t, hipEvent_t);
 
 struct TileEntry {
     int prec;
@@ -30,7 +33,7 @@ extern TileEntry kTiles[];
 extern const int kNumTiles;
 
 // Split-precision MFMA kernels (FP32, lengths 16^S).
-typedef hipError_t (*MfmaLaunch)(const MfmaArgs&, dim3, hipStream_t);
+typedef hipError_t (*MfmaLaunch)(const MfmaArgs&, dim3, hipStream_t, hipEvent_t, hipEvent_t);
 struct MfmaEntry {
     int L;
     int nb;

@@ -70,7 +70,9 @@ typedef enum gfftx_status {
 #define GFFTX_FLAG_SPLIT_MFMA 0x2u   /* FP32 C2C: radix-16 stages as split-precision MFMA */
 #define GFFTX_FLAG_NO_AUTOTUNE 0x4u  /* use the built-in schedule, ignore the tuning cache */
 #define GFFTX_FLAG_GRAPH 0x8u        /* replay execution through a captured hipGraph      */
-#define GFFTX_FLAG_PROFILE 0x10u     /* record HIP events around every kernel launch      */
+#define GFFTX_FLAG_PROFILE 0x10u     /* time every kernel launch with a HIP event pair
+                                        (FFT tile and MFMA kernels: events attached to the
+                                        dispatch; element-wise kernels: recorded around it) */
 
 typedef struct gfftx_plan_s* gfftx_plan;
 typedef struct gfftx_ragged_s* gfftx_ragged;
