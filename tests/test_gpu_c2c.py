@@ -396,6 +396,51 @@ def test_c2_config_2p24_round_trip(torch_cuda, dtype):
     assert e_rt <= 2 * TOL[dtype]
 
 
+@pytest.mark.parametrize("dtype,log2n", [(np.complex64, 28), (np.complex128, 26)])
+def test_max_length_properties(torch_cuda, dtype, log2n):
+    """The longest native lengths (FP32 2^28, FP64 2^26), where the oracle would take minutes,
+    through size-independent properties:
+      - a tone exp(2 pi i f j / N) transforms to N at bin f and zero elsewhere (sign, order
+        and scale of the forward transform);
+      - Parseval: sum |X|^2 = N sum |x|^2 on normal input;
+      - the inverse transform returns the input.
+    FP32 bar 5e-7: 2^28 adds four radix-2 levels of rounding to the 2^24 case (2e-7)."""
+    import math
+
+    torch = torch_cuda
+    n = 1 << log2n
+    tdt = torch.complex64 if dtype == np.complex64 else torch.complex128
+    tol = 5e-7 if dtype == np.complex64 else 1e-13
+    print(f"case max length 2^{log2n} {np.dtype(dtype).name}", flush=True)
+    f = 123457
+    j = torch.arange(n, dtype=torch.int64, device="cuda")
+    ph = ((j * f) % n).to(torch.float64) * (2 * math.pi / n)  # f j mod N is exact
+    del j
+    x = torch.polar(torch.ones_like(ph), ph).to(tdt)
+    del ph
+    with api.Plan(n, VT[dtype], api.DFT, api.IN_PLACE) as pf, \
+            api.Plan(n, VT[dtype], api.IDFT, api.IN_PLACE) as pi:
+        pf.fft(x)
+        torch.cuda.synchronize()
+        peak = complex(x[f].item())
+        x[f] -= n
+        e_tone = torch.linalg.vector_norm(x.to(torch.complex128)).item() / n
+        g = torch.Generator(device="cuda").manual_seed(log2n)
+        x = torch.randn(n, dtype=tdt, device="cuda", generator=g)
+        x0 = x.clone()
+        pf.fft(x)
+        e_pars = abs(torch.sum(torch.abs(x.to(torch.complex128)) ** 2).item()
+                     / (n * torch.sum(torch.abs(x0.to(torch.complex128)) ** 2).item()) - 1.0)
+        pi.fft(x)
+        e_rt = (torch.linalg.vector_norm((x - x0).to(torch.complex128)) /
+                torch.linalg.vector_norm(x0.to(torch.complex128))).item()
+    print(f"2^{log2n}: tone peak {peak} relL2 {e_tone:.3e}; Parseval {e_pars:.3e}; round trip {e_rt:.3e}")
+    assert abs(peak - n) <= tol * n
+    assert e_tone <= tol
+    assert e_pars <= 10 * tol
+    assert e_rt <= 2 * tol
+
+
 @pytest.mark.parametrize("radix,digits", [(2, 10), (2, 16), (3, 7), (5, 5), (7, 4)])
 def test_digit_reverse_bit_exact(torch_cuda, radix, digits):
     """The permutation kernel equals GFFTswap2's in-place digit reversal bit for bit."""
