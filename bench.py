@@ -41,9 +41,9 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 # rocprofv3 --pmc over `python bench.py --counters` (N = 2^24 FP32 pass kernels): average
 # FETCH_SIZE and WRITE_SIZE per dispatch in KiB, and the same counters for a 256 MiB device
 # copy (__amd_rocclr_copyBuffer) in that run.
-PMC_COPY_KIB = {"fetch": 131086.94, "write": 262144.0, "bytes_kib": 262144.0}
+PMC_COPY_KIB = {"fetch": 131086.83, "write": 262144.0, "bytes_kib": 262144.0}
 PMC_KIB = {
-    "tile_float_L256_nb16_wg256_truefalse": {"fetch": 65617.08, "write": 131072.0},
+    "tile_float_L256_nb16_wg256_truefalse": {"fetch": 65616.44, "write": 131072.0},
     "tile_float_L256_nb16_wg256_truetrue": {"fetch": 65749.36, "write": 131072.0},
 }
 
@@ -193,19 +193,14 @@ def round_trip_run(torch, api, n, dtype, steps, warmup, nbuf, seed=24):
     graph_ms = _call_time_us(torch, lambda: one(gf, gi), iters=steps) * 1e-3
     gf.close()
     gi.close()
-    # per-kernel durations (HIP events on the launch stream around every kernel)
+    # per-kernel durations (HIP events attached to every kernel dispatch), after settling
     qf = api.Plan(n, vt, api.DFT, api.IN_PLACE, flags=api.FLAG_PROFILE)
     qi = api.Plan(n, vt, api.IDFT, api.IN_PLACE, flags=api.FLAG_PROFILE)
-    for i in range(3):
-        qf.fft(bufs[i % nbuf])
-        qi.fft(bufs[i % nbuf])
-    torch.cuda.synchronize()
+    _settle(torch, lambda: one(qf, qi))
     qf.profile()
     qi.profile()
-    for i in range(min(steps, 50)):
-        b = bufs[i % nbuf]
-        qf.fft(b)
-        qi.fft(b)
+    for i in range(max(steps, 100)):
+        one(qf, qi)
     torch.cuda.synchronize()
     prof = {"fwd": qf.profile(), "inv": qi.profile()}
     qf.close()
