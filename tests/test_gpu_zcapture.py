@@ -1,0 +1,77 @@
+"""Plans executed inside a capture the caller runs (torch.cuda.graph on the caller's stream).
+
+The library records nothing but kernel launches, event records and stream waits on the
+stream it is given (no memsets, no allocations, no synchronisation for device pointers), so a
+caller may capture a call into its own graph.  Each replay runs after eager work and a
+synchronisation of its own and must equal the eager result bit for bit (same kernels)."""
+import numpy as np
+import pytest
+
+from gfft_amd import api, inputs
+
+pytestmark = pytest.mark.gpu
+
+
+def _check_replays(torch, call, bufs, x_devs, refs, reps=3):
+    """Captures call() once, then replays it `reps` times, each after refilling the inputs,
+    some unrelated eager work and a synchronisation; every replay must give refs."""
+    for b, x in zip(bufs, x_devs):
+        b.copy_(x)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        call()
+    junk = torch.ones(1 << 20, device="cuda")
+    for r in range(reps):
+        for b, x in zip(bufs, x_devs):
+            b.copy_(x)
+        junk.mul_(1.0001).add_(r)  # eager work between replays
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for out, ref in zip(bufs[len(bufs) - len(refs):], refs):
+            assert torch.equal(out, ref), f"replay {r}"
+    del g
+
+
+@pytest.mark.parametrize("case", ["multipass_f32", "batched_f64", "r2c_bins_f32", "chirp_z_f32", "ragged_f32"])
+def test_caller_capture_replays_match_eager(torch_cuda, case):
+    torch = torch_cuda
+    print(f"case {case}", flush=True)
+    if case == "ragged_f32":
+        lengths = [256, 4096, 1009, 3 * 2 ** 12, 2 ** 15, 64, 5120]
+        x = torch.from_numpy(inputs.complex_normal(sum(lengths), 90).astype(np.complex64)).cuda()
+        plan = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT)
+        ref = x.clone()
+        plan.fft(ref)
+        buf = torch.empty_like(x)
+        _check_replays(torch, lambda: plan.fft(buf), [buf], [x], [ref])
+        plan.close()
+        return
+    if case == "multipass_f32":
+        n, vt, tt, place, batch, flags, dt = 2 ** 20, api.COMPLEX_FLOAT, api.DFT, api.IN_PLACE, 1, 0, np.complex64
+    elif case == "batched_f64":
+        n, vt, tt, place, batch, flags, dt = 4096, api.COMPLEX_DOUBLE, api.IDFT, api.IN_PLACE, 64, 0, np.complex128
+    elif case == "r2c_bins_f32":
+        n, vt, tt, place, batch, flags, dt = 3 * 2 ** 12, api.FLOAT, api.RDFT, api.OUT_OF_PLACE, 1, \
+            api.FLAG_R2C_BINS, np.float32
+    else:
+        n, vt, tt, place, batch, flags, dt = 1009, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE, 3, 0, np.complex64
+    if dt == np.float32:
+        xs = inputs.complex_normal(n * batch, 91).view(np.float64).astype(np.float32)
+        out_shape, out_dt = (n + 1,), torch.complex64
+    else:
+        xs = inputs.complex_normal(n * batch, 91).astype(dt)
+        out_shape, out_dt = None, None
+    x = torch.from_numpy(xs).cuda()
+    with api.Plan(n, vt, tt, place, batch=batch, flags=flags) as p:
+        if place == api.IN_PLACE:
+            ref = x.clone()
+            p.fft(ref)
+            buf = torch.empty_like(x)
+            _check_replays(torch, lambda: p.fft(buf), [buf], [x], [ref])
+        else:
+            o_ref = torch.empty(out_shape, dtype=out_dt, device="cuda") if out_shape else torch.empty_like(x)
+            p.fft(x, o_ref)
+            src, dst = torch.empty_like(x), torch.empty_like(o_ref)
+            _check_replays(torch, lambda: p.fft(src, dst), [src, dst], [x], [o_ref])
