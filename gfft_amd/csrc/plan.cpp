@@ -207,6 +207,12 @@ bool prof_begin(gfftx_plan_s* p, size_t slot, const char* name, hipStream_t s, s
                 bool attached)
 {
     if (!(p->flags & GFFTX_FLAG_PROFILE) || p->capturing) return false;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;  // a capture the caller runs
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    if (cs != hipStreamCaptureStatusNone) return false;
     if (p->slots.size() <= slot) p->slots.resize(slot + 1);
     gfftx_plan_s::Slot& sl = p->slots[slot];
     sl.name = name;

@@ -75,3 +75,22 @@ def test_caller_capture_replays_match_eager(torch_cuda, case):
             p.fft(x, o_ref)
             src, dst = torch.empty_like(x), torch.empty_like(o_ref)
             _check_replays(torch, lambda: p.fft(src, dst), [src, dst], [x], [o_ref])
+
+
+def test_caller_capture_of_profiled_plan(torch_cuda):
+    """A GFFTX_FLAG_PROFILE plan records no timings while the caller captures its stream; its
+    replays still equal the eager result, and eager calls are timed again afterwards."""
+    torch = torch_cuda
+    n = 2 ** 18
+    x = torch.from_numpy(inputs.complex_normal(n, 92).astype(np.complex64)).cuda()
+    with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.IN_PLACE, flags=api.FLAG_PROFILE) as p:
+        ref = x.clone()
+        p.fft(ref)
+        torch.cuda.synchronize()
+        assert all(cnt == 1 for _, _, cnt in p.profile())
+        buf = torch.empty_like(x)
+        _check_replays(torch, lambda: p.fft(buf), [buf], [x], [ref])
+        assert all(cnt == 0 for _, _, cnt in p.profile())
+        p.fft(buf)
+        torch.cuda.synchronize()
+        assert all(cnt == 1 for _, _, cnt in p.profile())
