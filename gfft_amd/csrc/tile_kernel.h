@@ -374,7 +374,7 @@ __device__ __forceinline__ Ctx<C> make_ctx(const TileArgs& a, unsigned tile, uns
 
 // Transposed store of a tile whose load and store modes differ: the LDS holds the
 // natural-order outputs of the NB sequences.
-template <class C, bool IDLE> __device__ __forceinline__ void tile_finish(const Ctx<C>& x, unsigned tile, unsigned by)
+template <class C, bool IDLE> __device__ __forceinline__ void tile_finish(const Ctx<C>& x, unsigned tile)
 {
     using V = typename C::V;
     if constexpr (!C::DIRECT) {
@@ -454,7 +454,7 @@ template <class C, bool IDLE> __device__ __forceinline__ void tile_finish(const 
 // R2C split of a SplitCfg tile from the natural-order outputs in LDS (forward only):
 //   X[k] = (Z[k] + conj Z[N-k]) / 2 - i W_{2N}^k (Z[k] - conj Z[N-k]) / 2,
 // each element written by the thread that stores it, at its own position k = t + q S.
-template <class C, bool IDLE> __device__ __forceinline__ void split_finish(const Ctx<C>& x, unsigned tile, unsigned by)
+template <class C, bool IDLE> __device__ __forceinline__ void split_finish(const Ctx<C>& x, unsigned tile)
 {
     using V = typename C::V;
     using S_ = decltype(V{}.x);
@@ -516,8 +516,8 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, unsigned tile, unsi
     FirstIn<C> f;
     load_first<C, IDLE>(x, f);
     stage_loop<C, IDLE, 0>(x, f);
-    if constexpr (C::SPLIT) split_finish<C, IDLE>(x, tile, by);
-    else tile_finish<C, IDLE>(x, tile, by);
+    if constexpr (C::SPLIT) split_finish<C, IDLE>(x, tile);
+    else tile_finish<C, IDLE>(x, tile);
 }
 
 template <class C> __global__ __launch_bounds__(C::WG) void tile_kernel(TileArgs a)
