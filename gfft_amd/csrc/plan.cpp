@@ -457,6 +457,7 @@ int exec_graph(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
 
 int exec_any(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
 {
+    std::lock_guard<std::mutex> lk(p->mu);
     DeviceGuard g(p->device);
     if (!g.ok) return fail(GFFTX_ERR_DEVICE, "cannot select device %d", p->device);
     const bool din = is_device_ptr(in), dout = is_device_ptr(out);
@@ -649,6 +650,7 @@ int gfftx_plan_profile(gfftx_plan plan, double* total_ms, uint64_t* count, const
 {
     if (!plan || !total_ms || !count || max < 0) return fail(GFFTX_ERR_INVALID_ARG, "bad arguments");
     if (!(plan->flags & GFFTX_FLAG_PROFILE)) return fail(GFFTX_ERR_INVALID_ARG, "plan was not created with GFFTX_FLAG_PROFILE");
+    std::lock_guard<std::mutex> lk(plan->mu);
     DeviceGuard g(plan->device);
     const int n = (int)plan->slots.size();
     if (nslots) *nslots = n;

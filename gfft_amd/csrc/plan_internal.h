@@ -217,6 +217,9 @@ struct gfftx_plan_s {
     void* blue_chirp = nullptr;        // n entries W_{2n}^{j^2}
     void* blue_filter = nullptr;       // blue_m entries: spectrum of the chirp filter
     void* blue_work = nullptr;         // batch x blue_m
+    // calls from several threads run one at a time: the graph cache, the profiling slots and
+    // the host staging are the plan's own
+    std::mutex mu;
 };
 
 // ---------------------------------------------------------------------------------------
@@ -258,6 +261,7 @@ struct gfftx_ragged_s {
     void* stage = nullptr;
     // profiling (GFFTX_FLAG_PROFILE): events around the launch
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending, pool;
+    std::mutex mu;  // calls from several threads run one at a time
 };
 
 namespace gfftx {

@@ -546,6 +546,7 @@ int gfftx_ragged_exec(gfftx_ragged plan, void* data, void* stream)
 {
     if (!plan) return fail(GFFTX_ERR_INVALID_ARG, "plan is NULL");
     if (!data) return fail(GFFTX_ERR_INVALID_ARG, "data is NULL");
+    std::lock_guard<std::mutex> lk(plan->mu);
     DeviceGuard g(plan->device);
     if (!g.ok) return fail(GFFTX_ERR_DEVICE, "cannot select device %d", plan->device);
     hipStream_t s = (hipStream_t)stream;
@@ -574,6 +575,7 @@ int gfftx_ragged_profile(gfftx_ragged plan, double* total_ms, uint64_t* count)
 {
     if (!plan || !total_ms || !count) return fail(GFFTX_ERR_INVALID_ARG, "bad arguments");
     if (!(plan->flags & GFFTX_FLAG_PROFILE)) return fail(GFFTX_ERR_INVALID_ARG, "plan was not created with GFFTX_FLAG_PROFILE");
+    std::lock_guard<std::mutex> lk(plan->mu);
     DeviceGuard g(plan->device);
     double tot = 0;
     for (auto& ev : plan->pending) {

@@ -32,8 +32,11 @@
  * memory (the call copies through pinned staging buffers in chunks, overlapping the host
  * copies, both DMA directions and the transform of earlier batch items, and returns after
  * the result is back, as GFFT's synchronous fft() requires; GFFTX_STAGE_CHUNK_MB sets the
- * chunk, default 16).  `stream` is a hipStream_t; NULL is the null stream.  Calls that
- * share a plan must not overlap: a plan owns its scratch and staging buffers.
+ * chunk, default 16).  `stream` is a hipStream_t; NULL is the null stream.  Calls on one plan
+ * may come from several threads (GFFT's fft() is reentrant): the plan runs them one at a
+ * time, a host-pointer call until its result is back, a device-pointer call until its work
+ * is enqueued.  Device-pointer calls on one plan but different streams share the plan's
+ * scratch, so the caller must order those streams.
  * No function throws; each returns a gfftx_status.  gfftx_last_error() gives detail for the
  * calling thread.
  */

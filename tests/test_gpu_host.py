@@ -82,3 +82,54 @@ def test_host_call_matches_oracle_and_repeats(torch_cuda):
             out = np.zeros_like(x)
             p.fft(x, out)
             assert rel_l2(out, ref) <= 1e-13
+
+
+def test_threads_share_one_plan(torch_cuda):
+    """GFFT's fft() is reentrant: one transform object may be called from several threads.
+    Host-pointer calls from four threads on one plan (and on one ragged plan) each return
+    what a call alone returns, bit for bit."""
+    import threading
+
+    n, batch = 2 ** 16, 4
+    xs = [inputs.complex_normal(n * batch, 700 + i).astype(np.complex64) for i in range(4)]
+    lengths = [4096, 2 ** 15, 1536, 20480]
+    rs = [inputs.complex_normal(sum(lengths), 710 + i).astype(np.complex64) for i in range(4)]
+    with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE, batch=batch) as p:
+        rp = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT)
+        alone = []
+        for x in xs:
+            y = np.zeros_like(x)
+            p.fft(x, y)
+            alone.append(y)
+        ragged_alone = []
+        for r in rs:
+            z = r.copy()
+            rp.fft(z)
+            ragged_alone.append(z)
+        got = [[None] * 3 for _ in xs]
+        got_r = [[None] * 3 for _ in rs]
+        errors = []
+
+        def work(i):
+            try:
+                for k in range(3):
+                    y = np.zeros_like(xs[i])
+                    p.fft(xs[i], y)
+                    got[i][k] = y
+                    z = rs[i].copy()
+                    rp.fft(z)
+                    got_r[i][k] = z
+            except Exception as e:  # reported below; a thread's exception is not raised
+                errors.append(repr(e))
+
+        th = [threading.Thread(target=work, args=(i,)) for i in range(len(xs))]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        rp.close()
+    assert not errors, errors
+    for i in range(len(xs)):
+        for k in range(3):
+            assert np.array_equal(got[i][k], alone[i]), (i, k)
+            assert np.array_equal(got_r[i][k], ragged_alone[i]), (i, k)
