@@ -313,6 +313,12 @@ def other_workloads(torch, api):
         errs = np.linalg.norm(r - ref, axis=1) / np.linalg.norm(ref, axis=1)
         c5[label]["relL2_max_vs_fp64"] = float(errs.max())
     c5["mfma_over_plain"] = round(c5["plain"]["us"] / c5["mfma"]["us"], 3)
+    # matrix-core work per launch: 16384 transforms x 3 radix-16 stages x 16 blocks x 2 row
+    # halves x 3 hi/lo products of 16x16x32 (the instruction count the MFMA counters read)
+    mflops = 16384 * 3 * 16 * 2 * 3 * (16 * 16 * 32 * 2)
+    c5["mfma"]["mfma_flops_per_launch"] = mflops
+    c5["mfma"]["mfma_TFLOPs"] = round(mflops / c5["mfma"]["us"] / 1e6, 1)
+    c5["mfma"]["mfma_frac_of_fp16_peak"] = round(mflops / c5["mfma"]["us"] / 1e6 / 2500.0, 4)
     res["c5_16384x4096_f32"] = c5
     res["host_pointers"] = host_pointer_workloads(torch, api)
     # any-N path: a prime length (chirp-z convolution of smooth length m >= 2n-1), FP32
