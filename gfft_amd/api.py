@@ -35,7 +35,7 @@ EXPORTED = [
     "gfftx_plan_create", "gfftx_plan_create_batched", "gfftx_exec_inplace", "gfftx_exec_oop",
     "gfftx_plan_destroy", "gfftx_plan_describe", "gfftx_plan_profile", "gfftx_ragged_create", "gfftx_ragged_exec",
     "gfftx_ragged_destroy", "gfftx_ragged_describe", "gfftx_ragged_profile", "gfftx_digit_reverse",
-    "gfftx_strerror", "gfftx_last_error",
+    "gfftx_stream_synchronize", "gfftx_strerror", "gfftx_last_error",
 ]
 
 _lib = None
@@ -77,6 +77,7 @@ def lib():
         L.gfftx_ragged_describe.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
         L.gfftx_ragged_profile.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64)]
         L.gfftx_digit_reverse.argtypes = [vp, vp, u64, u32, ci, vp]
+        L.gfftx_stream_synchronize.argtypes = [vp]
         L.gfftx_strerror.argtypes = [ci]
         L.gfftx_strerror.restype = ctypes.c_char_p
         L.gfftx_last_error.restype = ctypes.c_char_p

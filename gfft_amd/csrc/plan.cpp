@@ -645,6 +645,13 @@ int gfftx_exec_oop(gfftx_plan plan, const void* src, void* dst, void* stream)
     return exec_any(plan, src, dst, (hipStream_t)stream);
 }
 
+int gfftx_stream_synchronize(void* stream)
+{
+    const hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+    if (e != hipSuccess) return fail(GFFTX_ERR_DEVICE, "stream synchronize: %s", hipGetErrorString(e));
+    return GFFTX_SUCCESS;
+}
+
 int gfftx_plan_profile(gfftx_plan plan, double* total_ms, uint64_t* count, const char** names, int max,
                        int* nslots)
 {

@@ -103,6 +103,12 @@ int gfftx_exec_inplace(gfftx_plan plan, void* data, void* stream);
  * AbstractFFT_oop<T>::fft (src/gfftpolicy.h:53-58). */
 int gfftx_exec_oop(gfftx_plan plan, const void* src, void* dst, void* stream);
 
+/* Wait until the work enqueued on `stream` (a hipStream_t; NULL: the null stream) has
+ * finished.  No reference counterpart: GFFT's fft() is synchronous
+ * (src/gfftpolicy.h:163-172, :203-212), and the C++ policy layer calls this after each
+ * device-pointer call to keep that guarantee. */
+int gfftx_stream_synchronize(void* stream);
+
 /* Release a plan.  Replaces: `delete` of the object CreateTransformObject returned
  * (virtual destructors, src/gfftpolicy.h:50,57). */
 int gfftx_plan_destroy(gfftx_plan plan);

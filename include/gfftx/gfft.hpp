@@ -187,7 +187,11 @@ struct Exec<VType, Type, IN_PLACE, Parall> : public AbstractFFT_inp<typename VTy
     typedef typename VType::ValueType T;
     PlanHolder h;
     explicit Exec(unsigned long n) : h(n, VType::ID, Type::ID, IN_PLACE::ID, Parall::Device) {}
-    void fft(T* data) override { check(gfftx_exec_inplace(h.get(), data, nullptr), "fft(T*)"); }
+    void fft(T* data) override
+    {
+        check(gfftx_exec_inplace(h.get(), data, nullptr), "fft(T*)");
+        check(gfftx_stream_synchronize(nullptr), "fft(T*)");
+    }
 };
 
 template <class VType, class Type, class Parall>
@@ -198,6 +202,7 @@ struct Exec<VType, Type, OUT_OF_PLACE, Parall> : public AbstractFFT_oop<typename
     void fft(const T* src, T* dst) override
     {
         check(gfftx_exec_oop(h.get(), src, dst, nullptr), "fft(const T*, T*)");
+        check(gfftx_stream_synchronize(nullptr), "fft(const T*, T*)");
     }
 };
 
@@ -339,6 +344,7 @@ public:
     void fft(typename VType::ValueType* data) override
     {
         check(gfftx_exec_inplace(h_.get(), data, nullptr), "fft(T*)");
+        check(gfftx_stream_synchronize(nullptr), "fft(T*)");
     }
 
 private:
@@ -351,6 +357,7 @@ public:
     void fft(const typename VType::ValueType* src, typename VType::ValueType* dst) override
     {
         check(gfftx_exec_oop(h_.get(), src, dst, nullptr), "fft(const T*, T*)");
+        check(gfftx_stream_synchronize(nullptr), "fft(const T*, T*)");
     }
 
 private:

@@ -197,6 +197,21 @@ def test_host_pointers_are_synchronous(torch_cuda):
         assert rel_l2(out, ref) <= 1e-13
 
 
+def test_stream_synchronize_waits_for_device_calls(torch_cuda):
+    """gfftx_stream_synchronize, which the C++ policy layer calls after every device-pointer
+    fft() to keep GFFT's synchronous semantics, returns only when the stream is idle."""
+    torch = torch_cuda
+    n = 2 ** 24
+    x = torch.randn(n, dtype=torch.complex64, device="cuda")
+    with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.IN_PLACE) as p:
+        p.fft(x, stream=0)
+        assert api.lib().gfftx_stream_synchronize(None) == api.SUCCESS
+        for _ in range(4):
+            p.fft(x, stream=0)  # about 0.5 ms of work on the null stream
+        assert api.lib().gfftx_stream_synchronize(None) == api.SUCCESS
+        assert torch.cuda.default_stream().query()
+
+
 @pytest.mark.parametrize("n,vt,tt", [(2 ** 20, api.COMPLEX_FLOAT, api.DFT), (4096, api.COMPLEX_DOUBLE, api.IDFT),
                                      (3 * 2 ** 12, api.FLOAT, api.RDFT), (2 ** 16, api.DOUBLE, api.IRDFT)])
 def test_graph_replays_match_eager(torch_cuda, n, vt, tt):
