@@ -72,6 +72,26 @@ def test_rdft_irdft(torch_cuda, dtype, n):
         assert np.array_equal(host(s), z)
 
 
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n,batch", [(64, 1), (4096, 3), (3 * 2 ** 10, 2), (2 ** 18, 1), (3 * 2 ** 19, 1)])
+def test_rdft_irdft_in_place(torch_cuda, dtype, n, batch):
+    """GFFT's real transforms are in place only (Separate, src/gfftswap.h:239-287; out of
+    place exits): RDFT in place against the long-double packed reference for every batch
+    item, then IRDFT in place back to the input."""
+    z = inputs.complex_normal(n * batch, 60 + n).astype(dtype).reshape(batch, n)
+    print(f"case real in place n={n} batch={batch} dtype={np.dtype(dtype).name}", flush=True)
+    with api.Plan(n, VT[dtype], api.RDFT, api.IN_PLACE, batch=batch) as pf, \
+            api.Plan(n, VT[dtype], api.IRDFT, api.IN_PLACE, batch=batch) as pi:
+        d = dev(torch_cuda, z)
+        pf.fft(d)
+        y = host(d)
+        for b in range(batch):
+            ref, _ = packed_ref(z[b])
+            assert rel_l2(y[b], ref) <= TOL[dtype], b
+        pi.fft(d)
+        assert rel_l2(host(d), z) <= 2 * TOL[dtype]
+
+
 @pytest.mark.parametrize("dtype,n,passes,batch,layout", [
     (np.complex64, 2 ** 15, "128,256", 3, "nr"), (np.complex64, 49152, "192,256", 1, "bins"),
     (np.complex64, 46080, "45,1024", 2, "nr"), (np.complex64, 2 ** 18, "1024,256", 1, "bins"),
