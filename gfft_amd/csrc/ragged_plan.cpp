@@ -455,9 +455,12 @@ int ragged_launch(gfftx_ragged_s* r, void* data, hipStream_t s)
     const bool prof = (r->flags & GFFTX_FLAG_PROFILE) != 0;
     if (prof) {
         if (r->pool.empty()) {
-            hipEvent_t x, y;
-            HIP_TRY(hipEventCreate(&x));
-            HIP_TRY(hipEventCreate(&y));
+            hipEvent_t x = nullptr, y = nullptr;
+            if (hipEventCreate(&x) != hipSuccess || hipEventCreate(&y) != hipSuccess) {
+                if (x) (void)hipEventDestroy(x);
+                (void)hipGetLastError();
+                return fail(GFFTX_ERR_DEVICE, "cannot create profiling events");
+            }
             r->pool.push_back({x, y});
         }
         ev = r->pool.back();

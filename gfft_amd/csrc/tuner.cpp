@@ -99,9 +99,12 @@ int time_plan(gfftx_plan_s* p, void* a, void* b, hipStream_t s, int reps, float*
         if (rc) return rc;
     }
     std::vector<float> t;
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        (void)hipGetLastError();
+        return fail(GFFTX_ERR_DEVICE, "cannot create timing events");
+    }
     for (int g = 0; g < 3; ++g) {
         (void)hipEventRecord(e0, s);
         int rc = GFFTX_SUCCESS;
