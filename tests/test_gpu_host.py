@@ -39,6 +39,8 @@ CASES = [
     (2 ** 15, 9, api.FLOAT, api.IRDFT, api.OUT_OF_PLACE),          # real inverse (copy of src)
     (1009, 300, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE),      # chirp-z, many groups
     (5 * 2 ** 17, 2, api.COMPLEX_DOUBLE, api.DFT, api.OUT_OF_PLACE),
+    (2 ** 14, 5, api.DOUBLE, api.RDFT, api.IN_PLACE),              # GFFT's real mode: in place
+    (3 * 2 ** 18, 2, api.FLOAT, api.IRDFT, api.IN_PLACE),           # multi-pass real inverse
 ]
 
 
@@ -70,6 +72,28 @@ def test_host_call_equals_device_call(torch_cuda, n, batch, vt, tt, place):
             p.fft(x, o2)
             assert np.array_equal(host(o2), ref)
             assert np.array_equal(host(d), x)
+
+
+def test_host_call_split_mfma(torch_cuda):
+    """The split-precision MFMA plan called on host arrays returns what its device call
+    returns, bit for bit (forward out of place, inverse in place)."""
+    n, batch = 4096, 64
+    x = inputs.complex_normal(n * batch, 650).astype(np.complex64).reshape(batch, n)
+    with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE, batch=batch, flags=api.FLAG_SPLIT_MFMA) as pf, \
+            api.Plan(n, api.COMPLEX_FLOAT, api.IDFT, api.IN_PLACE, batch=batch, flags=api.FLAG_SPLIT_MFMA) as pi:
+        d = dev(torch_cuda, x)
+        o = torch_cuda.empty_like(d)
+        pf.fft(d, o)
+        ref = host(o)
+        ho = np.zeros_like(x)
+        pf.fft(x, ho)
+        assert np.array_equal(ho, ref)
+        pi.fft(o)
+        back = host(o)
+        hb = ho.copy()
+        pi.fft(hb)
+        assert np.array_equal(hb, back)
+        assert rel_l2(hb, x) <= 4e-7
 
 
 def test_host_call_matches_oracle_and_repeats(torch_cuda):
