@@ -480,3 +480,35 @@ def test_cpp_policy_program(torch_cuda):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     print(r.stdout[-2000:])
     assert r.returncode == 0 and "ALL OK" in r.stdout, r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_every_length_to_256(torch_cuda, dtype):
+    """GFFT's out-of-place path takes any N (src/gfftalg.h:459-507): every length from 2 to 256
+    (products of 2, 3, 5, 7 natively, the others through the chirp-z path), forward and
+    inverse, three transforms per call, against the long-double O(N^2) DFT."""
+    def smooth(m):
+        for p in (2, 3, 5, 7):
+            while m % p == 0:
+                m //= p
+        return m == 1
+
+    worst = (0.0, 0)
+    for n in range(2, 257):
+        # chirp-z lengths: FP32 bar 5e-7 (two length-m transforms and three chirp products)
+        tol = TOL[dtype] if dtype == np.complex128 or smooth(n) else 5e-7
+        x = inputs.complex_normal(3 * n, 900 + n).astype(dtype).reshape(3, n)
+        ref = np.stack([O.dft_ld(x[b].astype(np.complex128), 1) for b in range(3)])
+        with api.Plan(n, VT[dtype], api.DFT, api.OUT_OF_PLACE, batch=3) as pf, \
+                api.Plan(n, VT[dtype], api.IDFT, api.OUT_OF_PLACE, batch=3) as pi:
+            s = dev(torch_cuda, x)
+            o = torch_cuda.empty_like(s)
+            pf.fft(s, o)
+            y = host(o)
+            e = rel_l2(y, ref)
+            assert e <= tol, (n, e)
+            worst = max(worst, (e, n))
+            back = torch_cuda.empty_like(s)
+            pi.fft(o, back)
+            assert rel_l2(host(back), x) <= 2 * tol, n
+    print(f"every length 2..256 {np.dtype(dtype).name}: worst relL2 {worst[0]:.2e} at n={worst[1]}")
