@@ -219,3 +219,34 @@ def test_radix8_pass_tiles(torch_cuda, monkeypatch, n, passes, tnb, twg):
             d = dev(torch_cuda, z)
             p.fft(d)
             assert rel_l2(host(d), want) <= 2e-7
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_real_every_even_length_to_256(torch_cuda, dtype):
+    """RDFT and IRDFT in place for every even complex length n from 2 to 256 (2n reals; n
+    with a prime factor above 7 through the chirp-z path), NR packing, against the
+    long-double O(N^2) DFT of the 2n reals."""
+    def smooth(m):
+        for p in (2, 3, 5, 7):
+            while m % p == 0:
+                m //= p
+        return m == 1
+
+    worst = (0.0, 0)
+    for n in range(2, 257, 2):
+        tol = TOL[dtype] if dtype == np.complex128 or smooth(n) else 5e-7
+        z = inputs.complex_normal(n, 950 + n).astype(dtype)
+        r = np.stack([z.real, z.imag], 1).ravel().astype(np.complex128)
+        X = O.dft_ld(r, 1)
+        ref = X[:n].copy()
+        ref[0] = X[0].real + 1j * X[n].real
+        with api.Plan(n, VT[dtype], api.RDFT, api.IN_PLACE) as pf, \
+                api.Plan(n, VT[dtype], api.IRDFT, api.IN_PLACE) as pi:
+            d = dev(torch_cuda, z)
+            pf.fft(d)
+            e = rel_l2(host(d), ref)
+            assert e <= tol, (n, e)
+            worst = max(worst, (e, n))
+            pi.fft(d)
+            assert rel_l2(host(d), z) <= 2 * tol, n
+    print(f"real every even length 2..256 {np.dtype(dtype).name}: worst relL2 {worst[0]:.2e} at n={worst[1]}")
