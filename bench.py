@@ -321,6 +321,7 @@ def other_workloads(torch, api):
     c5["mfma"]["mfma_frac_of_fp16_peak"] = round(mflops / c5["mfma"]["us"] / 1e6 / 2500.0, 4)
     res["c5_16384x4096_f32"] = c5
     res["host_pointers"] = host_pointer_workloads(torch, api)
+    res["vs_torch_fft"] = vendor_fft_comparison(torch, api)
     # any-N path: a prime length (chirp-z convolution of smooth length m >= 2n-1), FP32
     n = 1048573  # prime
     xp = torch.from_numpy(inputs.complex_normal(n, 13).astype(np.complex64)).cuda()
@@ -339,6 +340,56 @@ def other_workloads(torch, api):
                                         "kernel_events_sum_us": round(sum(t for _, t in ks), 1)}
     del xp, op
     return res
+
+
+def vendor_fft_comparison(torch, api):
+    """The same transforms through torch.fft (hipFFT over the rocFFT that PyTorch ships), timed
+    the same way (calls back to back between two events, after settling): the vendor library
+    on the same device, as a reference point.  torch.fft writes out of place."""
+    out = {}
+    n = 1 << 24
+    bufs = [torch.randn(n, dtype=torch.complex64, device="cuda") for _ in range(4)]
+    tmp = torch.empty_like(bufs[0])
+    ctr = [0]
+
+    def torch_rt():
+        b = bufs[ctr[0] % 4]
+        ctr[0] += 1
+        torch.fft.fft(b, out=tmp)
+        torch.fft.ifft(tmp, out=b)
+
+    with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.IN_PLACE) as pf, \
+            api.Plan(n, api.COMPLEX_FLOAT, api.IDFT, api.IN_PLACE) as pi:
+        def ours_rt():
+            b = bufs[ctr[0] % 4]
+            ctr[0] += 1
+            pf.fft(b)
+            pi.fft(b)
+
+        t_torch = _call_time_us(torch, torch_rt)
+        t_ours = _call_time_us(torch, ours_rt)
+    out["c2_f32_round_trip"] = {"torch_fft_us": round(t_torch, 1), "gfftx_us": round(t_ours, 1),
+                                "speedup": round(t_torch / t_ours, 2)}
+    del bufs, tmp
+    torch.cuda.empty_cache()
+    x5 = torch.randn(16384, 4096, dtype=torch.complex64, device="cuda")
+    o5 = torch.empty_like(x5)
+    t_torch = _call_time_us(torch, lambda: torch.fft.fft(x5, dim=-1, out=o5), iters=20)
+    with api.Plan(4096, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE, batch=16384) as p:
+        t_ours = _call_time_us(torch, lambda: p.fft(x5, o5), iters=20)
+    out["c5_16384x4096_f32"] = {"torch_fft_us": round(t_torch, 1), "gfftx_us": round(t_ours, 1),
+                                "speedup": round(t_torch / t_ours, 2)}
+    del x5, o5
+    torch.cuda.empty_cache()
+    nr = 3 * 2 ** 20
+    z = torch.randn(nr, dtype=torch.float32, device="cuda")
+    oz = torch.empty(nr // 2 + 1, dtype=torch.complex64, device="cuda")
+    t_torch = _call_time_us(torch, lambda: torch.fft.rfft(z, out=oz), iters=200)
+    with api.Plan(nr // 2, api.FLOAT, api.RDFT, api.OUT_OF_PLACE, flags=api.FLAG_R2C_BINS) as p:
+        t_ours = _call_time_us(torch, lambda: p.fft(z, oz), iters=200)
+    out["c3_r2c_3x2p20_f32"] = {"torch_fft_us": round(t_torch, 1), "gfftx_us": round(t_ours, 1),
+                                "speedup": round(t_torch / t_ours, 2)}
+    return out
 
 
 def host_pointer_workloads(torch, api, reps=5):
