@@ -73,3 +73,50 @@ def test_cpp_policy_header_compiles():
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), src],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_header_is_plain_c(tmp_path):
+    """include/gfftx.h is a C header (the boundary any FFI binds): a C99 translation unit that
+    calls every entry point compiles and links against the library with gcc."""
+    src = tmp_path / "use_gfftx.c"
+    src.write_text(r"""
+#include <stdio.h>
+#include "gfftx.h"
+int main(void)
+{
+    gfftx_plan p = 0;
+    gfftx_ragged r = 0;
+    uint64_t lens[2] = {256, 512};
+    double ms[4];
+    uint64_t cnt[4];
+    const char* names[4];
+    int nslots = 0;
+    char buf[64];
+    int rc = gfftx_plan_create(&p, 1024, GFFTX_COMPLEX_FLOAT, GFFTX_DFT, GFFTX_IN_PLACE, 0, 0);
+    if (rc == GFFTX_SUCCESS) {
+        (void)gfftx_plan_describe(p, buf, sizeof buf);
+        (void)gfftx_plan_profile(p, ms, cnt, names, 4, &nslots);
+        (void)gfftx_exec_inplace(p, 0, 0);
+        (void)gfftx_exec_oop(p, 0, 0, 0);
+        (void)gfftx_plan_destroy(p);
+    }
+    rc = gfftx_plan_create_batched(&p, 1024, 2, GFFTX_COMPLEX_DOUBLE, GFFTX_IDFT, GFFTX_OUT_OF_PLACE, 0, 0);
+    if (rc == GFFTX_SUCCESS) (void)gfftx_plan_destroy(p);
+    rc = gfftx_ragged_create(&r, lens, 2, GFFTX_COMPLEX_FLOAT, GFFTX_DFT, 0, 0);
+    if (rc == GFFTX_SUCCESS) {
+        (void)gfftx_ragged_exec(r, 0, 0);
+        (void)gfftx_ragged_describe(r, buf, sizeof buf);
+        (void)gfftx_ragged_profile(r, ms, cnt);
+        (void)gfftx_ragged_destroy(r);
+    }
+    (void)gfftx_digit_reverse(0, 0, 2, 4, GFFTX_COMPLEX_FLOAT, 0);
+    (void)gfftx_stream_synchronize(0);
+    printf("%s | %s\n", gfftx_strerror(rc), gfftx_last_error());
+    return 0;
+}
+""")
+    exe = tmp_path / "use_gfftx"
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src),
+                        "-L", os.path.dirname(api.LIB_PATH), "-lgfftx", "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
