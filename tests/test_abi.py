@@ -120,3 +120,12 @@ int main(void)
                         "-L", os.path.dirname(api.LIB_PATH), "-lgfftx", "-o", str(exe)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_cpp_policy_header_compiles_cleanly():
+    """include/gfftx/gfft.hpp (GFFT's names and the HIP<Dev> policy) compiles without
+    warnings under -Wall -Wextra, as the C++ check program that drives it."""
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-fsyntax-only", "-I",
+                        os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "cpp", "gfft_check.cpp")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
