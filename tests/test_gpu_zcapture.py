@@ -1,4 +1,5 @@
-"""Plans executed inside a capture the caller runs (torch.cuda.graph on the caller's stream).
+"""Plans executed inside a capture the caller runs (torch.cuda.graph on the caller's stream),
+and plans issued on several streams at once.
 
 The library records nothing but kernel launches, event records and stream waits on the
 stream it is given (no memsets, no allocations, no synchronisation for device pointers), so a
@@ -94,3 +95,38 @@ def test_caller_capture_of_profiled_plan(torch_cuda):
         p.fft(buf)
         torch.cuda.synchronize()
         assert all(cnt == 1 for _, _, cnt in p.profile())
+
+
+def test_plans_on_two_streams_match_serial(torch_cuda):
+    """Calls on different plans may run concurrently: two plans issued alternately on two
+    streams (default priority) give, bit for bit, what the same calls give one after the
+    other on one stream."""
+    torch = torch_cuda
+    a = torch.from_numpy(inputs.complex_normal(2 ** 20, 93).astype(np.complex64)).cuda()
+    b = torch.from_numpy(inputs.complex_normal(4096 * 64, 94).astype(np.complex128)).cuda()
+    with api.Plan(2 ** 20, api.COMPLEX_FLOAT, api.DFT, api.IN_PLACE) as fa, \
+            api.Plan(2 ** 20, api.COMPLEX_FLOAT, api.IDFT, api.IN_PLACE) as ia, \
+            api.Plan(4096, api.COMPLEX_DOUBLE, api.DFT, api.IN_PLACE, batch=64) as fb, \
+            api.Plan(4096, api.COMPLEX_DOUBLE, api.IDFT, api.IN_PLACE, batch=64) as ib:
+        ra, rb = a.clone(), b.clone()
+        for _ in range(3):
+            fa.fft(ra)
+            ia.fft(ra)
+            fb.fft(rb)
+            ib.fft(rb)
+        torch.cuda.synchronize()
+        ca, cb = a.clone(), b.clone()
+        torch.cuda.synchronize()
+        s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+        for _ in range(3):
+            with torch.cuda.stream(s1):
+                fa.fft(ca)
+            with torch.cuda.stream(s2):
+                fb.fft(cb)
+            with torch.cuda.stream(s1):
+                ia.fft(ca)
+            with torch.cuda.stream(s2):
+                ib.fft(cb)
+        torch.cuda.synchronize()
+        assert torch.equal(ca, ra)
+        assert torch.equal(cb, rb)
