@@ -494,6 +494,11 @@ def main():
                 "kernel": dom_key, "avg_us": round(dom_avg_s * 1e6, 2),
                 "bytes_per_launch": bytes_per_launch,
                 "copy_GBps": round(copy_bps / 1e9, 1), "frac_of_copy": round(achieved / copy_bps, 4)}
+    # every pass of the timed round trips together: each pass reads and writes the array once
+    npasses = 2 * len(json.loads(r32["plan"])["passes"])
+    all_bps = npasses * bytes_per_launch / (ms_step * 1e-3)
+    roofline["all_passes_GBps"] = round(all_bps / 1e9, 1)
+    roofline["all_passes_frac"] = round(all_bps / HBM_PEAK, 4)
     per_kernel = {k: {"avg_us": round(v[0] / v[1] * 1e3, 2),
                       "GBps_alg": round(bytes_per_launch / (v[0] / v[1] * 1e-3) / 1e9, 1)}
                   for k, v in kern.items()}
