@@ -273,18 +273,19 @@ def other_workloads(torch, api):
                                 "kernels": [(k, round(t, 1)) for k, t in ks],
                                 "kernel_events_sum_us": round(sum(t for _, t in ks), 1)}
     del z, o
-    # C4: ragged batch of 4096 FP32 transforms, one persistent launch
+    # C4: ragged batch of 4096 FP32 transforms, one call replayed as a captured hipGraph (the
+    # workload's definition); the same calls issued directly are reported beside it
     lengths = [int(v) for v in inputs.ragged_lengths(4096, 4096)]
     tot = sum(lengths)
     xg = torch.from_numpy(inputs.complex_normal(tot, 4096).astype(np.complex64)).cuda()
-    rp = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT)
+    rp = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT, flags=api.FLAG_GRAPH)
     us = _call_time_us(torch, lambda: rp.fft(xg), iters=30)
     rp.close()
-    rp = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT, flags=api.FLAG_GRAPH)
-    us_graph = _call_time_us(torch, lambda: rp.fft(xg), iters=30)
+    rp = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT)
+    us_direct = _call_time_us(torch, lambda: rp.fft(xg), iters=30)
     rp.close()
     flops = sum(5 * L * np.log2(L) for L in lengths)
-    res["c4_ragged_4096_f32"] = {"us": round(us, 1), "graph_us": round(us_graph, 1),
+    res["c4_ragged_4096_f32"] = {"us": round(us, 1), "launch": "hipGraph replay", "direct_us": round(us_direct, 1),
                                  "GFLOPs": round(flops / us / 1e3, 1),
                                  "GBps_alg": round(2 * tot * 8 / us / 1e3, 1), "points": tot}
     del xg
