@@ -254,6 +254,24 @@ def other_workloads(torch, api):
     res["c1_fp64_1024_x65536"] = {"us": round(us, 1), "GFLOPs": round(5 * n * 10 * batch / us / 1e3, 1),
                                   "GBps_alg": round(2 * n * batch * 16 / us / 1e3, 1), "kernels": kernel}
     del x
+    # C1 single transform: time per call issued back to back, and the latency of one forward
+    # call waited for (launch, kernel and synchronisation, host clock, median of 200); forward
+    # and inverse calls alternate so that the values stay bounded
+    x1 = torch.from_numpy(inputs.complex_uniform(n, 17)).cuda()
+    with api.Plan(n, api.DOUBLE, api.DFT, api.IN_PLACE) as p, \
+            api.Plan(n, api.DOUBLE, api.IDFT, api.IN_PLACE) as q:
+        us1 = _call_time_us(torch, lambda: (p.fft(x1), q.fft(x1)), iters=200) / 2
+        lat = []
+        for _ in range(200):
+            t0 = time.perf_counter()
+            p.fft(x1)
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - t0)
+            q.fft(x1)
+            torch.cuda.synchronize()
+    res["c1_fp64_1024_single"] = {"us_per_call_back_to_back": round(us1, 2),
+                                  "synchronous_call_us_median": round(float(np.median(lat)) * 1e6, 1)}
+    del x1
     # C3: R2C N = 3*2^20 real FP32 -> N/2+1 bins
     nr = 3 * 2 ** 20
     xr, _, _ = inputs.real_sinusoids(nr, 3)
