@@ -6,11 +6,18 @@ plain-method workload), inputs resident in HBM, rotating over buffers totalling 
 reference's flop count 5 N log2 N per direction (test/gfft_performance.h:77).
 
 Also reported on the same line:
-  roofline      dominant kernel's algorithmic bytes (one read + one write of the array,
-                2 N b_c per pass) over its average duration from HIP events recorded on
-                the launch stream (GFFTX_FLAG_PROFILE), against 8 TB/s;
-  cpu_baseline  the GFFT restatement (OpenMP in-place path) on the host cores, bounded
-                sample, same input distribution;
+  roofline      algorithmic bytes of the timed step (2 N b_c per transform: one read and one
+                write of the array) over the step's time, against 8 TB/s; `dominant_kernel`
+                gives the longest launch's share of those bytes (2 N b_c / passes) over its
+                average duration from HIP events attached to its dispatches
+                (GFFTX_FLAG_PROFILE), and `per_pass_*` the bytes one pass moves (2 N b_c)
+                over the same duration;
+  errors        every workload's forward result against numpy's complex128 FFT of the same
+                (rounded) input, computed outside the timed regions: relative L2, largest
+                absolute component error and GFFT's max|err|/max|y| over re and im
+                components (test/gfft_accuracy.h:190-203);
+  cpu_baseline  the GFFT restatement (OpenMP in-place path) on the host cores, FP32 and FP64
+                round trips, bounded sample, same input;
   fp64          the same round trip in FP64.
 
 roofline.traffic is the HBM traffic of the dominant kernel per launch from two rocprofv3
@@ -56,6 +63,26 @@ def hbm_traffic(kernel_name):
     fetch = c["fetch"] * PMC_COPY_KIB["bytes_kib"] / PMC_COPY_KIB["fetch"]
     write = c["write"] * PMC_COPY_KIB["bytes_kib"] / PMC_COPY_KIB["write"]
     return int(round((fetch + write) * 1024))
+
+
+def errors(y, ref):
+    """Errors of a computed transform y against an FP64 reference: relL2 = |y-ref|_2/|ref|_2,
+    max_abs = largest |re| or |im| error component, rel_inf = max_abs / (largest |re| or |im|
+    component of y), GFFT's relative error (test/gfft_accuracy.h:190-203)."""
+    y = np.asarray(y).astype(np.complex128).ravel()
+    ref = np.asarray(ref).astype(np.complex128).ravel()
+    d = y - ref
+    dmax = float(max(np.abs(d.real).max(), np.abs(d.imag).max()))
+    ymax = float(max(np.abs(y.real).max(), np.abs(y.imag).max()))
+    return {"relL2": float(np.linalg.norm(d) / np.linalg.norm(ref)), "max_abs": dmax,
+            "rel_inf": dmax / ymax if ymax > 0 else 0.0}
+
+
+def np_fft_rows(x, inverse=False):
+    """numpy (pocketfft) complex128 FFT of each row: the independent FP64 reference, with the
+    engine's conventions (forward e^-, inverse e^+ scaled 1/N)."""
+    x = np.asarray(x).astype(np.complex128)
+    return np.fft.ifft(x, axis=-1) if inverse else np.fft.fft(x, axis=-1)
 
 
 def copy_rate(torch, nbytes=1 << 30, iters=20):
@@ -153,11 +180,14 @@ def round_trip_run(torch, api, n, dtype, steps, warmup, nbuf, seed=24):
     pf = api.Plan(n, vt, api.DFT, api.IN_PLACE)
     pi = api.Plan(n, vt, api.IDFT, api.IN_PLACE)
     stream = torch.cuda.current_stream()
-    # one round trip: the identity up to rounding
+    # forward result against numpy's complex128 FFT of the same input, then the round trip
+    # (the identity up to rounding)
     pf.fft(bufs[0])
+    torch.cuda.synchronize()
+    fwd_err = errors(bufs[0].cpu().numpy(), np_fft_rows(x))
     pi.fft(bufs[0])
     torch.cuda.synchronize()
-    rt_err = float(np.linalg.norm(bufs[0].cpu().numpy() - x) / np.linalg.norm(x))
+    rt_err = errors(bufs[0].cpu().numpy(), x)
     bufs[0].copy_(torch.from_numpy(x))
     ctr = [0]
 
@@ -207,7 +237,7 @@ def round_trip_run(torch, api, n, dtype, steps, warmup, nbuf, seed=24):
     qi.close()
     del bufs
     torch.cuda.empty_cache()
-    return {"ms": ms, "wall_s": wall, "profile": prof, "rt_err": rt_err, "plan": desc,
+    return {"ms": ms, "wall_s": wall, "profile": prof, "rt_err": rt_err, "fwd_err": fwd_err, "plan": desc,
             "graph_ms_per_step": graph_ms}
 
 
@@ -247,19 +277,29 @@ def other_workloads(torch, api):
     res = {}
     # C1: N=2^10 FP64 in place forward, 2^16 copies per call
     n, batch = 1024, 65536
-    x = torch.from_numpy(inputs.complex_uniform(n * batch, 17)).cuda()
+    x_h = inputs.complex_uniform(n * batch, 17)
+    x = torch.from_numpy(x_h).cuda()
     with api.Plan(n, api.DOUBLE, api.DFT, api.IN_PLACE, batch=batch) as p:
         us = _call_time_us(torch, lambda: p.fft(x), iters=30)
         kernel = [k["kernel"] for k in json.loads(p.describe())["passes"]]
+        x.copy_(torch.from_numpy(x_h))  # one call on the original input for the errors
+        p.fft(x)
+        rows = x.view(batch, n)[::16].cpu().numpy()
     res["c1_fp64_1024_x65536"] = {"us": round(us, 1), "GFLOPs": round(5 * n * 10 * batch / us / 1e3, 1),
-                                  "GBps_alg": round(2 * n * batch * 16 / us / 1e3, 1), "kernels": kernel}
-    del x
+                                  "GBps_alg": round(2 * n * batch * 16 / us / 1e3, 1), "kernels": kernel,
+                                  "errors_vs_numpy_fp64": errors(rows, np_fft_rows(x_h.reshape(batch, n)[::16])),
+                                  "errors_rows": "every 16th of the 65536 transforms"}
+    del x, x_h, rows
     # C1 single transform: time per call issued back to back, and the latency of one forward
     # call waited for (launch, kernel and synchronisation, host clock, median of 200); forward
     # and inverse calls alternate so that the values stay bounded
-    x1 = torch.from_numpy(inputs.complex_uniform(n, 17)).cuda()
+    x1_h = inputs.complex_uniform(n, 17)
+    x1 = torch.from_numpy(x1_h).cuda()
     with api.Plan(n, api.DOUBLE, api.DFT, api.IN_PLACE) as p, \
             api.Plan(n, api.DOUBLE, api.IDFT, api.IN_PLACE) as q:
+        p.fft(x1)
+        e1 = errors(x1.cpu().numpy(), np_fft_rows(x1_h))
+        x1.copy_(torch.from_numpy(x1_h))
         us1 = _call_time_us(torch, lambda: (p.fft(x1), q.fft(x1)), iters=200) / 2
         lat = []
         for _ in range(200):
@@ -270,7 +310,8 @@ def other_workloads(torch, api):
             q.fft(x1)
             torch.cuda.synchronize()
     res["c1_fp64_1024_single"] = {"us_per_call_back_to_back": round(us1, 2),
-                                  "synchronous_call_us_median": round(float(np.median(lat)) * 1e6, 1)}
+                                  "synchronous_call_us_median": round(float(np.median(lat)) * 1e6, 1),
+                                  "errors_vs_numpy_fp64": e1}
     del x1
     # C3: R2C N = 3*2^20 real FP32 -> N/2+1 bins
     nr = 3 * 2 ** 20
@@ -280,6 +321,8 @@ def other_workloads(torch, api):
     # whole-call time of calls issued back to back, then of graph replays; the per-kernel
     # event pairs (which each include a launch) give the breakdown
     with api.Plan(nr // 2, api.FLOAT, api.RDFT, api.OUT_OF_PLACE, flags=api.FLAG_R2C_BINS) as pd:
+        pd.fft(z, o)
+        e3 = errors(o.cpu().numpy(), np.fft.rfft(xr.astype(np.float32).astype(np.float64)))
         us = _call_time_us(torch, lambda: pd.fft(z, o), iters=200)
     with api.Plan(nr // 2, api.FLOAT, api.RDFT, api.OUT_OF_PLACE, flags=api.FLAG_R2C_BINS | api.FLAG_GRAPH) as pg:
         us_graph = _call_time_us(torch, lambda: pg.fft(z, o), iters=200)
@@ -289,14 +332,19 @@ def other_workloads(torch, api):
                                 "GFLOPs_2.5NlogN": round(2.5 * nr * np.log2(nr) / us / 1e3, 1),
                                 "GBps_alg": round((4 * nr + 8 * (nr // 2 + 1)) / us / 1e3, 1),
                                 "kernels": [(k, round(t, 1)) for k, t in ks],
-                                "kernel_events_sum_us": round(sum(t for _, t in ks), 1)}
+                                "kernel_events_sum_us": round(sum(t for _, t in ks), 1),
+                                "errors_vs_numpy_fp64": e3}
     del z, o
     # C4: ragged batch of 4096 FP32 transforms, one call replayed as a captured hipGraph (the
     # workload's definition); the same calls issued directly are reported beside it
     lengths = [int(v) for v in inputs.ragged_lengths(4096, 4096)]
     tot = sum(lengths)
-    xg = torch.from_numpy(inputs.complex_normal(tot, 4096).astype(np.complex64)).cuda()
+    xg_h = inputs.complex_normal(tot, 4096).astype(np.complex64)
+    xg = torch.from_numpy(xg_h).cuda()
     rp = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT, flags=api.FLAG_GRAPH)
+    rp.fft(xg)
+    e4 = ragged_errors(xg.cpu().numpy(), xg_h, lengths)
+    xg.copy_(torch.from_numpy(xg_h))
     us = _call_time_us(torch, lambda: rp.fft(xg), iters=30)
     rp.close()
     rp = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT)
@@ -305,8 +353,9 @@ def other_workloads(torch, api):
     flops = sum(5 * L * np.log2(L) for L in lengths)
     res["c4_ragged_4096_f32"] = {"us": round(us, 1), "launch": "hipGraph replay", "direct_us": round(us_direct, 1),
                                  "GFLOPs": round(flops / us / 1e3, 1),
-                                 "GBps_alg": round(2 * tot * 8 / us / 1e3, 1), "points": tot}
-    del xg
+                                 "GBps_alg": round(2 * tot * 8 / us / 1e3, 1), "points": tot,
+                                 "errors_vs_numpy_fp64": e4}
+    del xg, xg_h
     # C5: 16384 x 4096 FP32 plain vs split-precision MFMA; accuracy against the FP64 engine
     n, batch = 4096, 16384
     xs = inputs.sinusoid_batch(batch, n, 5)
@@ -318,19 +367,18 @@ def other_workloads(torch, api):
                       flags=flags) as p:
             us = _call_time_us(torch, lambda: p.fft(x32, out), iters=30)
             kernel = json.loads(p.describe())["passes"][0]["kernel"]
-            rows = out.cpu().numpy()[::64]
+            rows = out.view(batch, n)[::16].cpu().numpy()
         c5[label] = {"kernel": kernel, "us": round(us, 1),
                      "GFLOPs": round(5 * n * 12 * batch / us / 1e3, 1),
                      "GBps_alg": round(2 * n * batch * 8 / us / 1e3, 1), "rows": rows}
     del x32, out
-    x64 = torch.from_numpy(xs[::64].astype(np.complex64).astype(np.complex128)).cuda()
-    with api.Plan(n, api.COMPLEX_DOUBLE, api.DFT, api.IN_PLACE, batch=x64.shape[0]) as p:
-        p.fft(x64)
-    ref = x64.cpu().numpy()
+    ref = np_fft_rows(xs[::16].astype(np.complex64))
     for label in ("plain", "mfma"):
         r = c5[label].pop("rows")
         errs = np.linalg.norm(r - ref, axis=1) / np.linalg.norm(ref, axis=1)
-        c5[label]["relL2_max_vs_fp64"] = float(errs.max())
+        c5[label]["errors_vs_numpy_fp64"] = errors(r, ref)
+        c5[label]["relL2_max_per_transform"] = float(errs.max())
+    c5["errors_rows"] = "every 16th of the 16384 transforms"
     c5["mfma_over_plain"] = round(c5["plain"]["us"] / c5["mfma"]["us"], 3)
     # matrix-core work per launch: 16384 transforms x 3 radix-16 stages x 16 blocks x 2 row
     # halves x 3 hi/lo products of 16x16x32 (the instruction count the MFMA counters read)
@@ -343,9 +391,12 @@ def other_workloads(torch, api):
     res["vs_torch_fft"] = vendor_fft_comparison(torch, api)
     # any-N path: a prime length (chirp-z convolution of smooth length m >= 2n-1), FP32
     n = 1048573  # prime
-    xp = torch.from_numpy(inputs.complex_normal(n, 13).astype(np.complex64)).cuda()
+    xp_h = inputs.complex_normal(n, 13).astype(np.complex64)
+    xp = torch.from_numpy(xp_h).cuda()
     op = torch.empty_like(xp)
     with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE) as pd:
+        pd.fft(xp, op)
+        ez = errors(op.cpu().numpy(), np_fft_rows(xp_h))
         us = _call_time_us(torch, lambda: pd.fft(xp, op), iters=100)
     with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE, flags=api.FLAG_GRAPH) as pg:
         us_graph = _call_time_us(torch, lambda: pg.fft(xp, op), iters=100)
@@ -356,9 +407,32 @@ def other_workloads(torch, api):
                                         "GFLOPs_5NlogN": round(5 * n * np.log2(n) / us / 1e3, 1),
                                         "GBps_caller": round(2 * n * 8 / us / 1e3, 1),
                                         "kernels": len(ks),
-                                        "kernel_events_sum_us": round(sum(t for _, t in ks), 1)}
-    del xp, op
+                                        "kernel_events_sum_us": round(sum(t for _, t in ks), 1),
+                                        "errors_vs_numpy_fp64": ez}
+    del xp, op, xp_h
     return res
+
+
+def ragged_errors(y, x, lengths):
+    """Errors of a ragged batch against numpy's complex128 FFT of every transform: the worst
+    per-transform relL2 and the errors over the whole batch."""
+    offs = np.concatenate([[0], np.cumsum(lengths)])
+    ref = np.empty(len(y), np.complex128)
+    worst = 0.0
+    by_len = {}
+    for i, L in enumerate(lengths):
+        by_len.setdefault(L, []).append(i)
+    for L, idx in by_len.items():
+        rows = np.stack([x[offs[i]:offs[i] + L] for i in idx])
+        r = np_fft_rows(rows)
+        got = np.stack([y[offs[i]:offs[i] + L] for i in idx]).astype(np.complex128)
+        rel = np.linalg.norm(got - r, axis=1) / np.linalg.norm(r, axis=1)
+        worst = max(worst, float(rel.max()))
+        for j, i in enumerate(idx):
+            ref[offs[i]:offs[i] + L] = r[j]
+    out = errors(y, ref)
+    out["relL2_max_per_transform"] = worst
+    return out
 
 
 def vendor_fft_comparison(torch, api):
@@ -448,29 +522,55 @@ def host_pointer_workloads(torch, api, reps=5):
     return out
 
 
+def cpu_info():
+    """(CPU model, logical CPUs this process may run on, CPUs the machine reports)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    return model, allowed, os.cpu_count() or 1
+
+
 def cpu_baseline(n_log2: int):
-    """GFFT restatement, OpenMP in-place path (src/gfftomp.h:105-144), FP32 round trip of
-    2^n_log2 points, best of 3 (test/gfft_performance.h:104-123), wall clock."""
+    """GFFT restatement, OpenMP in-place path (src/gfftomp.h:105-144): forward + inverse round
+    trips of 2^n_log2 points in FP32 and in FP64, best of 3 each (test/gfft_performance.h:
+    104-123), wall clock.  NT = the CPUs this process may run on, rounded down to a power of
+    two (GFFT's split uses gcd(N, NT) threads), at most 16: one GPU's share of a GPU host."""
     from gfft_amd import inputs
     from oracle import oracle as O
 
-    ncores = os.cpu_count() or 1
+    model, allowed, reported = cpu_info()
     nt = 1
-    while nt * 2 <= min(ncores, 16):
+    while nt * 2 <= min(allowed, 16):
         nt *= 2
     n = 1 << n_log2
-    x = inputs.complex_normal(n, 24).astype(np.complex64)
-    best = 1e30
-    for _ in range(3):
-        t0 = time.perf_counter()
-        y = O.gfft_inplace(x, 1, nt)
-        O.gfft_inplace(y, -1, nt)
-        best = min(best, time.perf_counter() - t0)
+    x = inputs.complex_normal(n, 24)
     flops = 2 * 5 * n * n_log2
-    return {"value": round(flops / best / 1e9, 3), "unit": "GFLOP/s", "cores": nt, "kind": "port",
-            "sample": f"3 round trips (forward+inverse, in place) of N=2^{n_log2} FP32 with the "
-                      f"GFFT OpenMP restatement (NT={nt}, serial combine and swap), best of 3 "
-                      f"= {best:.2f} s"}
+    res = {}
+    for label, dt in (("f32", np.complex64), ("f64", np.complex128)):
+        xx = x.astype(dt)
+        best = 1e30
+        for _ in range(3):
+            t0 = time.perf_counter()
+            y = O.gfft_inplace(xx, 1, nt)
+            O.gfft_inplace(y, -1, nt)
+            best = min(best, time.perf_counter() - t0)
+        res[label] = (flops / best / 1e9, best)
+    return {"value": round(res["f32"][0], 3), "unit": "GFLOP/s", "cores": nt, "kind": "port",
+            "fp64": {"value": round(res["f64"][0], 3), "unit": "GFLOP/s", "round_trip_s": round(res["f64"][1], 3)},
+            "cpu_model": model, "cpus_allowed": allowed, "cpus_reported": reported,
+            "sample": f"3 round trips (forward+inverse, in place) of N=2^{n_log2}, FP32 (value) and FP64 "
+                      f"(fp64), with the GFFT OpenMP restatement (NT={nt}, serial combine and swap), "
+                      f"best of 3 = {res['f32'][1]:.2f} s / {res['f64'][1]:.2f} s per round trip"}
 
 
 def main():
@@ -504,22 +604,33 @@ def main():
     dom_key = max(kern, key=lambda k: kern[k][0])
     dom_tot, dom_cnt = kern[dom_key]
     dom_avg_s = dom_tot / dom_cnt * 1e-3
-    bytes_per_launch = 2 * n * 8
-    achieved = bytes_per_launch / dom_avg_s
+    npasses = len(json.loads(r32["plan"])["passes"])
+    bytes_transform = 2 * n * 8  # algorithmic bytes of one transform: one read + one write
+    bytes_step = 2 * bytes_transform  # forward + inverse
+    achieved = bytes_step / (ms_step * 1e-3)
     copy_bps = copy_rate(torch)
+    # the longest launch: its share of the transform's algorithmic bytes (one of `npasses`
+    # passes), and the bytes a pass itself reads and writes (the whole array once)
+    dom_alg = bytes_transform / npasses / dom_avg_s
+    dom_pass = bytes_transform / dom_avg_s
     roofline = {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
                 "traffic": hbm_traffic(dom_key.split(":")[-1]) if args.log2n == 24 else None,
-                "kernel": dom_key, "avg_us": round(dom_avg_s * 1e6, 2),
-                "bytes_per_launch": bytes_per_launch,
+                "definition": "2 N 8 B per transform (one read + one write of the array), forward + "
+                              "inverse per step, over the timed step",
+                "dominant_kernel": {"kernel": dom_key, "avg_us": round(dom_avg_s * 1e6, 2),
+                                    "alg_bytes_per_launch": bytes_transform // npasses,
+                                    "alg_GBps": round(dom_alg / 1e9, 1), "alg_frac": round(dom_alg / HBM_PEAK, 4),
+                                    "pass_bytes_per_launch": bytes_transform,
+                                    "per_pass_GBps": round(dom_pass / 1e9, 1),
+                                    "per_pass_frac": round(dom_pass / HBM_PEAK, 4)},
                 "copy_GBps": round(copy_bps / 1e9, 1), "frac_of_copy": round(achieved / copy_bps, 4)}
     # every pass of the timed round trips together: each pass reads and writes the array once
-    npasses = 2 * len(json.loads(r32["plan"])["passes"])
-    all_bps = npasses * bytes_per_launch / (ms_step * 1e-3)
+    all_bps = 2 * npasses * bytes_transform / (ms_step * 1e-3)
     roofline["all_passes_GBps"] = round(all_bps / 1e9, 1)
     roofline["all_passes_frac"] = round(all_bps / HBM_PEAK, 4)
     per_kernel = {k: {"avg_us": round(v[0] / v[1] * 1e3, 2),
-                      "GBps_alg": round(bytes_per_launch / (v[0] / v[1] * 1e-3) / 1e9, 1)}
+                      "per_pass_GBps": round(bytes_transform / (v[0] / v[1] * 1e-3) / 1e9, 1)}
                   for k, v in kern.items()}
 
     out = {
@@ -540,22 +651,33 @@ def main():
                    "n": n, "passes": len(json.loads(r32["plan"])["passes"]),
                    "split": [p_["L"] for p_ in json.loads(r32["plan"])["passes"]],
                    "launch": "kernels issued directly on one stream",
-                   "graph_replay_ms_per_step": round(r32["graph_ms_per_step"], 4),
-                   "round_trip_relL2": r32["rt_err"]},
-        "alg_GBps": round(2 * 2 * n * 8 / (ms_step * 1e-3) / 1e9, 1),
+                   "graph_replay_ms_per_step": round(r32["graph_ms_per_step"], 4)},
+        "errors": {"forward_vs_numpy_fp64": r32["fwd_err"], "round_trip": r32["rt_err"],
+                   "input": "seed 24, FP32-rounded, reference numpy complex128 FFT"},
+        "alg_GBps": round(achieved / 1e9, 1),
         "roofline": roofline,
         "kernels": per_kernel,
     }
     if not args.no_others:
         out["workloads"] = other_workloads(torch, api)
+        c5 = out["workloads"]["c5_16384x4096_f32"]
+        # the metric's named comparison: split-precision MFMA against plain FP32 on the
+        # 16384 x 4096 batch, both errors against numpy's complex128 FFT
+        out["mfma_vs_plain_f32"] = {"speed_ratio": c5["mfma_over_plain"], "target_ratio": 1.5,
+                                    "relL2_gate": 2e-7,
+                                    "mfma_relL2": c5["mfma"]["errors_vs_numpy_fp64"]["relL2"],
+                                    "plain_relL2": c5["plain"]["errors_vs_numpy_fp64"]["relL2"],
+                                    "mfma_us": c5["mfma"]["us"], "plain_us": c5["plain"]["us"]}
     if not args.no_fp64:
         nbuf64 = max(2, math.ceil(512 * 2 ** 20 / (n * 16)))
         r64 = round_trip_run(torch, api, n, np.complex128, max(10, args.steps // 2),
                              args.warmup, nbuf64)
         ms64 = r64["ms"] / max(10, args.steps // 2)
         out["fp64"] = {"GFLOPs": round(flops_step / (ms64 * 1e-3) / 1e9, 2),
-                       "ms_per_step": round(ms64, 4), "round_trip_relL2": r64["rt_err"],
-                       "alg_GBps": round(2 * 2 * n * 16 / (ms64 * 1e-3) / 1e9, 1)}
+                       "ms_per_step": round(ms64, 4),
+                       "errors": {"forward_vs_numpy_fp64": r64["fwd_err"], "round_trip": r64["rt_err"]},
+                       "alg_GBps": round(2 * 2 * n * 16 / (ms64 * 1e-3) / 1e9, 1),
+                       "alg_frac": round(2 * 2 * n * 16 / (ms64 * 1e-3) / HBM_PEAK, 4)}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.log2n)
     print(json.dumps(out), flush=True)

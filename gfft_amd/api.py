@@ -110,6 +110,40 @@ def _ptr(x):
     raise TypeError(f"unsupported buffer type {type(x)}")
 
 
+def _scalar_bytes(x):
+    """Bytes of one real scalar of a float/complex torch tensor or numpy array, else None."""
+    if isinstance(x, np.ndarray):
+        if x.dtype.kind == "f":
+            return x.dtype.itemsize
+        if x.dtype.kind == "c":
+            return x.dtype.itemsize // 2
+        return None
+    dt = getattr(x, "dtype", None)
+    if dt is None or not (dt.is_floating_point or dt.is_complex):
+        return None
+    es = x.element_size()
+    return es // 2 if dt.is_complex else es
+
+
+def _check_buffer(x, need: int, fp64: bool, device: int, what: str):
+    """Rejects a buffer the plan would read or write past, or whose element type or device
+    does not match the plan (raw integer pointers are passed through unchecked)."""
+    if isinstance(x, int):
+        return
+    sb = _scalar_bytes(x)
+    want = 8 if fp64 else 4
+    if sb is None:
+        raise ValueError(f"{what}: float or complex data expected, got {getattr(x, 'dtype', type(x))}")
+    if sb != want:
+        raise ValueError(f"{what}: the plan computes in {'FP64' if fp64 else 'FP32'} "
+                         f"({want}-byte scalars), the buffer holds {sb}-byte scalars")
+    nbytes = x.nbytes if isinstance(x, np.ndarray) else x.numel() * x.element_size()
+    if nbytes < need:
+        raise ValueError(f"{what}: buffer of {nbytes} bytes, the plan needs {need}")
+    if hasattr(x, "is_cuda") and x.is_cuda and x.device.index != device:
+        raise ValueError(f"{what}: tensor on cuda:{x.device.index}, plan on device {device}")
+
+
 def _stream(x, stream):
     if stream is not None:
         return stream if isinstance(stream, int) else stream.cuda_stream
@@ -118,6 +152,19 @@ def _stream(x, stream):
 
         return torch.cuda.current_stream(x.device).cuda_stream
     return None
+
+
+def plan_bytes(n: int, vtype: int, ttype: int, flags: int = 0, batch: int = 1):
+    """(source, destination, in-place) bytes a plan reads or writes, the library's own rule:
+    dist complex elements per batch item (n, or n+1 for the bins layout of a real transform);
+    the real side of an out-of-place real transform holds n per item, its last item ending
+    at n complex elements."""
+    real = ttype in (RDFT, IRDFT)
+    eb = 16 if vtype in (DOUBLE, COMPLEX_DOUBLE) else 8
+    dist = n + (1 if real and (flags & FLAG_R2C_BINS) else 0)
+    full = dist * batch * eb
+    real_side = (dist * (batch - 1) + n) * eb
+    return (real_side if ttype == RDFT else full, real_side if ttype == IRDFT else full, full)
 
 
 class Plan:
@@ -135,13 +182,18 @@ class Plan:
                                                  device, flags)
         _check(rc, f"plan_create(n={n}, vtype={vtype}, ttype={ttype}, place={place}, batch={batch})")
         self._h = h
+        self.fp64 = vtype in (DOUBLE, COMPLEX_DOUBLE)
+        self.in_bytes, self.out_bytes, self.inplace_bytes = plan_bytes(n, vtype, ttype, flags, batch)
 
     def fft(self, data, dst=None, stream=None):
         """fft(data) in place, or fft(src, dst) out of place."""
         if dst is None:
+            _check_buffer(data, self.inplace_bytes, self.fp64, self.device, "fft(data)")
             rc = lib().gfftx_exec_inplace(self._h, _ptr(data), _stream(data, stream))
             _check(rc, "exec_inplace")
         else:
+            _check_buffer(data, self.in_bytes, self.fp64, self.device, "fft(src, dst): src")
+            _check_buffer(dst, self.out_bytes, self.fp64, self.device, "fft(src, dst): dst")
             rc = lib().gfftx_exec_oop(self._h, _ptr(data), _ptr(dst), _stream(data, stream))
             _check(rc, "exec_oop")
         return data if dst is None else dst
@@ -190,8 +242,12 @@ class RaggedPlan:
                                          flags), "ragged_create")
         self._h = h
         self.lengths = list(int(v) for v in lengths)
+        self.device = device
+        self.fp64 = vtype in (DOUBLE, COMPLEX_DOUBLE)
+        self.nbytes = sum(self.lengths) * (16 if self.fp64 else 8)
 
     def fft(self, data, stream=None):
+        _check_buffer(data, self.nbytes, self.fp64, self.device, "ragged fft(data)")
         _check(lib().gfftx_ragged_exec(self._h, _ptr(data), _stream(data, stream)), "ragged_exec")
         return data
 

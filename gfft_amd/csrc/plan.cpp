@@ -270,7 +270,7 @@ int run_passes(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, Range 
             ma.conj_in = p->inverse ? 1 : 0;
             ma.conj_out = p->inverse ? 1 : 0;
             ma.scale = p->inverse ? (float)(1.0 / (double)p->n) : 1.0f;
-            dim3 grid((unsigned)((ps.ntrans + ps.mfma->nb - 1) / ps.mfma->nb));
+            dim3 grid((unsigned)((ma.ntrans + ps.mfma->nb - 1) / ps.mfma->nb));
             std::pair<hipEvent_t, hipEvent_t> ev;
             const bool on = prof_begin(prof, slot0 + i, ps.mfma->name, s, &ev, true);
             hipError_t e = ps.mfma->launch(ma, grid, s, on ? ev.first : nullptr, on ? ev.second : nullptr);
@@ -451,15 +451,12 @@ int exec_graph(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
         g->out = out;
         g->used = ++p->graphs.tick;
     }
-    HIP_TRY(hipGraphLaunch(g->exec, s));
+    HIP_TRY(GraphCache::launch(*g, s));
     return GFFTX_SUCCESS;
 }
 
-int exec_any(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
+int exec_dispatch(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
 {
-    std::lock_guard<std::mutex> lk(p->mu);
-    DeviceGuard g(p->device);
-    if (!g.ok) return fail(GFFTX_ERR_DEVICE, "cannot select device %d", p->device);
     const bool din = is_device_ptr(in), dout = is_device_ptr(out);
     if (din && dout && (p->flags & GFFTX_FLAG_GRAPH)) return exec_graph(p, in, out, s);
     // complex elements the caller's arrays hold: the real side of a real transform has n
@@ -473,6 +470,23 @@ int exec_any(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
     }
     if (din && dout) return exec_device(p, in, out, s);
     return exec_host(p, in, out, s, din, dout, in_bytes, out_bytes);
+}
+
+// One call of the plan: calls from several threads run one at a time, and a call on another
+// stream than the previous call's waits for that call's work on the device (the scratch
+// arrays are shared).
+int exec_any(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
+{
+    std::lock_guard<std::mutex> lk(p->mu);
+    DeviceGuard g(p->device);
+    if (!g.ok) return fail(GFFTX_ERR_DEVICE, "cannot select device %d", p->device);
+    HIP_TRY(p->order.before(s));
+    const int rc = exec_dispatch(p, in, out, s);
+    // recorded also after a failed call, which may have enqueued part of its work
+    const hipError_t oe = p->order.after(s);
+    if (rc) return rc;
+    if (oe != hipSuccess) return fail(GFFTX_ERR_DEVICE, "stream order event: %s", hipGetErrorString(oe));
+    return GFFTX_SUCCESS;
 }
 
 // Moves the scratch buffers of `src` into `dst` where both have one (same length, batch and
@@ -686,6 +700,7 @@ int gfftx_plan_destroy(gfftx_plan plan)
     if (!plan) return GFFTX_SUCCESS;
     DeviceGuard g(plan->device);
     plan->graphs.clear();
+    plan->order.clear();
     for (int k = 0; k < 2; ++k) {
         if (plan->pin_in[k]) (void)hipHostFree(plan->pin_in[k]);
         if (plan->pin_out[k]) (void)hipHostFree(plan->pin_out[k]);

@@ -104,6 +104,7 @@ struct GraphCache {
         void* out = nullptr;
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
+        hipEvent_t done = nullptr;  // recorded after each launch of `exec`
         unsigned long long used = 0;
     };
     Entry e[kSlots];
@@ -133,17 +134,77 @@ struct GraphCache {
         release(*v);
         return v;
     }
+    // waits for the entry's last launch before its executable graph is destroyed
     static void release(Entry& x)
     {
+        if (x.done) {
+            (void)hipEventSynchronize(x.done);
+            (void)hipEventDestroy(x.done);
+        }
         if (x.exec) (void)hipGraphExecDestroy(x.exec);
         if (x.graph) (void)hipGraphDestroy(x.graph);
+        (void)hipGetLastError();
         x = Entry{};
+    }
+    // launches the entry's graph on s and records its completion event
+    static hipError_t launch(Entry& x, hipStream_t s)
+    {
+        if (!x.done) {
+            const hipError_t e = hipEventCreateWithFlags(&x.done, hipEventDisableTiming);
+            if (e != hipSuccess) return e;
+        }
+        const hipError_t e = hipGraphLaunch(x.exec, s);
+        if (e != hipSuccess) return e;
+        return hipEventRecord(x.done, s);
     }
     void clear()
     {
         for (auto& x : e) release(x);
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
         cap_stream = nullptr;
+    }
+};
+
+// Orders the calls of one plan that arrive on different streams.  A plan's scratch arrays,
+// staging buffers and graphs serve every call, so a call issued on another stream than the
+// previous one first waits (on the device, hipStreamWaitEvent) for the previous call's work.
+// Calls on one stream are ordered by the stream and pay one event record each.  A call on a
+// stream that the caller is capturing into a graph neither waits nor records: the caller
+// orders the replays of its own graph.
+struct StreamOrder {
+    hipEvent_t ev = nullptr;
+    hipStream_t last = nullptr;
+    bool valid = false;  // ev holds the previous call's completion
+    bool capturing = false;
+    // before a call on s; sets `capturing` for after()
+    hipError_t before(hipStream_t s)
+    {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        hipError_t e = hipStreamIsCapturing(s, &cs);
+        if (e != hipSuccess) return e;
+        capturing = cs != hipStreamCaptureStatusNone;
+        if (capturing || !valid || s == last) return hipSuccess;
+        return hipStreamWaitEvent(s, ev, 0);
+    }
+    // after the call's work has been enqueued on s
+    hipError_t after(hipStream_t s)
+    {
+        if (capturing) return hipSuccess;
+        if (!ev) {
+            const hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+            if (e != hipSuccess) return e;
+        }
+        const hipError_t e = hipEventRecord(ev, s);
+        if (e != hipSuccess) return e;
+        last = s;
+        valid = true;
+        return hipSuccess;
+    }
+    void clear()
+    {
+        if (ev) (void)hipEventDestroy(ev);
+        ev = nullptr;
+        valid = false;
     }
 };
 
@@ -220,6 +281,7 @@ struct gfftx_plan_s {
     // calls from several threads run one at a time: the graph cache, the profiling slots and
     // the host staging are the plan's own
     std::mutex mu;
+    StreamOrder order;  // calls from several streams wait for each other on the device
 };
 
 // ---------------------------------------------------------------------------------------
@@ -262,6 +324,7 @@ struct gfftx_ragged_s {
     // profiling (GFFTX_FLAG_PROFILE): events around the launch
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending, pool;
     std::mutex mu;  // calls from several threads run one at a time
+    StreamOrder order;  // calls from several streams wait for each other on the device
 };
 
 namespace gfftx {

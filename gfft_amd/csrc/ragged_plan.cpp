@@ -491,7 +491,7 @@ int ragged_launch(gfftx_ragged_s* r, void* data, hipStream_t s)
             ge->out = data;
             ge->used = ++r->graphs.tick;
         }
-        HIP_TRY(hipGraphLaunch(ge->exec, s));
+        HIP_TRY(GraphCache::launch(*ge, s));
     } else {
         const int lrc = ragged_launch_all(r, a, s);
         if (lrc) return lrc;
@@ -500,6 +500,22 @@ int ragged_launch(gfftx_ragged_s* r, void* data, hipStream_t s)
         HIP_TRY(hipEventRecord(ev.second, s));
         r->pending.push_back(ev);
     }
+    return GFFTX_SUCCESS;
+}
+
+int ragged_exec_any(gfftx_ragged_s* plan, void* data, hipStream_t s)
+{
+    if (is_device_ptr(data)) return ragged_launch(plan, data, s);
+    const size_t bytes = (size_t)plan->total * elem_bytes(plan->prec);
+    if (!plan->stage) {
+        int rc = dev_alloc_list(plan->allocs, &plan->stage, bytes);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipMemcpyAsync(plan->stage, data, bytes, hipMemcpyHostToDevice, s));
+    int rc = ragged_launch(plan, plan->stage, s);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(data, plan->stage, bytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
     return GFFTX_SUCCESS;
 }
 
@@ -545,6 +561,8 @@ int gfftx_ragged_create(gfftx_ragged* plan, const uint64_t* lengths, uint64_t co
     return GFFTX_SUCCESS;
 }
 
+// Calls from several threads run one at a time; a call on another stream than the previous
+// call's waits for that call's work on the device (the side-stream scratch is shared).
 int gfftx_ragged_exec(gfftx_ragged plan, void* data, void* stream)
 {
     if (!plan) return fail(GFFTX_ERR_INVALID_ARG, "plan is NULL");
@@ -553,17 +571,11 @@ int gfftx_ragged_exec(gfftx_ragged plan, void* data, void* stream)
     DeviceGuard g(plan->device);
     if (!g.ok) return fail(GFFTX_ERR_DEVICE, "cannot select device %d", plan->device);
     hipStream_t s = (hipStream_t)stream;
-    if (is_device_ptr(data)) return ragged_launch(plan, data, s);
-    const size_t bytes = (size_t)plan->total * elem_bytes(plan->prec);
-    if (!plan->stage) {
-        int rc = dev_alloc_list(plan->allocs, &plan->stage, bytes);
-        if (rc) return rc;
-    }
-    HIP_TRY(hipMemcpyAsync(plan->stage, data, bytes, hipMemcpyHostToDevice, s));
-    int rc = ragged_launch(plan, plan->stage, s);
+    HIP_TRY(plan->order.before(s));
+    const int rc = ragged_exec_any(plan, data, s);
+    const hipError_t oe = plan->order.after(s);
     if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(data, plan->stage, bytes, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    if (oe != hipSuccess) return fail(GFFTX_ERR_DEVICE, "stream order event: %s", hipGetErrorString(oe));
     return GFFTX_SUCCESS;
 }
 
@@ -599,6 +611,7 @@ int gfftx_ragged_destroy(gfftx_ragged plan)
     if (!plan) return GFFTX_SUCCESS;
     DeviceGuard g(plan->device);
     plan->graphs.clear();
+    plan->order.clear();
     for (auto& lc : plan->longs)
         if (lc.blue) gfftx_plan_destroy(lc.blue);
     for (int k = 0; k < kRaggedSides; ++k) {

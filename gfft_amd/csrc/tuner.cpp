@@ -1,13 +1,17 @@
 // tuner.cpp -- plan creation with the measured autotuner and its versioned JSON cache.
 #include "plan_internal.h"
 
+#include <fcntl.h>
+#include <sys/file.h>
+
 namespace gfftx {
 namespace detail {
 
 // ---------------------------------------------------------------------------------------
 // Autotuner: for a multi-pass plan, time the best few pass splits of the cost model on this
-// device and keep the fastest.  Results are cached per (device, precision, n, batch) in
-// memory and in a versioned JSON file ($GFFTX_TUNE_CACHE, default ~/.cache/gfftx/tune.json).
+// device and keep the fastest.  Results are cached per (device, precision, n, batch, kind,
+// place) in memory and in a versioned JSON file ($GFFTX_TUNE_CACHE, default
+// ~/.cache/gfftx/tune.json).
 std::mutex g_tune_mu;
 std::map<std::string, std::vector<int>> g_tune;
 std::string g_tune_loaded;  // cache file the in-memory entries belong to
@@ -21,12 +25,8 @@ std::string tune_path()
 
 // One entry per line: "key": [pass lengths..., 0, nb, wg per pass] (version 1 files hold
 // the pass lengths only, which still load)
-void tune_load()
+void read_entries(const std::string& path, std::map<std::string, std::vector<int>>& out)
 {
-    const std::string path = tune_path();
-    if (g_tune_loaded == path) return;
-    g_tune_loaded = path;
-    g_tune.clear();
     FILE* f = fopen(path.c_str(), "r");
     if (!f) return;
     char line[512];
@@ -50,11 +50,23 @@ void tune_load()
         std::vector<int> lens;
         std::vector<std::pair<int, int>> tiles;
         split_choice(v, lens, tiles);
-        if (lens.size() >= 2) g_tune[key] = v;
+        if (lens.size() >= 2) out[key] = v;
     }
     fclose(f);
 }
 
+void tune_load()
+{
+    const std::string path = tune_path();
+    if (g_tune_loaded == path) return;
+    g_tune_loaded = path;
+    g_tune.clear();
+    read_entries(path, g_tune);
+}
+
+// Writes the in-memory entries merged with what the file holds now: another process may have
+// stored entries since this one read it.  An exclusive lock on "<file>.lock" serialises the
+// read-merge-rename of concurrent writers; entries of this process win on equal keys.
 void tune_store()
 {
     const std::string path = tune_path();
@@ -64,9 +76,17 @@ void tune_store()
         for (size_t i = 1; i <= dir.size(); ++i)
             if (i == dir.size() || dir[i] == '/') (void)mkdir(dir.substr(0, i).c_str(), 0755);
     }
+    const int lk = open((path + ".lock").c_str(), O_CREAT | O_RDWR, 0644);
+    if (lk >= 0) (void)flock(lk, LOCK_EX);
+    std::map<std::string, std::vector<int>> disk;
+    read_entries(path, disk);
+    for (const auto& kv : disk) g_tune.insert(kv);  // keeps this process's entry on equal keys
     const std::string tmp = path + ".tmp" + std::to_string((long)getpid());
     FILE* f = fopen(tmp.c_str(), "w");
-    if (!f) return;
+    if (!f) {
+        if (lk >= 0) close(lk);
+        return;
+    }
     fprintf(f, "{\"version\": 2, \"entries\": {\n");
     size_t i = 0;
     for (const auto& kv : g_tune) {
@@ -77,6 +97,7 @@ void tune_store()
     fprintf(f, "}}\n");
     fclose(f);
     (void)rename(tmp.c_str(), path.c_str());
+    if (lk >= 0) close(lk);  // releases the lock
 }
 
 std::string tune_key(int device, int prec, uint64_t n, uint64_t batch)
@@ -139,9 +160,11 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
     if ((p->passes.size() < 2 && !single_pass) || (flags & GFFTX_FLAG_NO_AUTOTUNE) ||
         env_int("GFFTX_AUTOTUNE", 1) == 0 || getenv("GFFTX_PASSES"))
         return GFFTX_SUCCESS;
-    // a real forward plan is timed with the R2C split fused into its last pass, so its choice
-    // is kept apart from the complex transforms of the same length
-    const std::string key = tune_key(device, p->prec, n, batch) + (ttype == GFFTX_RDFT ? "|rdft" : "");
+    // a real forward plan is timed with the R2C split fused into its last pass, and every
+    // candidate runs with the plan's own placement (one buffer or two), so the choice is kept
+    // per transform kind and place; forward and inverse complex transforms share it
+    const std::string key = tune_key(device, p->prec, n, batch) + (ttype == GFFTX_RDFT ? "|rdft" : "") +
+                            (place == GFFTX_IN_PLACE ? "|ip" : "|oop");
     std::vector<int> chosen;
     std::string note;
     {

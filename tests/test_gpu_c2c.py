@@ -392,23 +392,29 @@ def test_very_long_fp32(torch_cuda):
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
 def test_c2_config_2p24_round_trip(torch_cuda, dtype):
-    """N=2^24 forward then inverse in place, N(0,1) seed 24: forward against the FP64 oracle,
-    round trip against the input."""
+    """N=2^24 forward then inverse in place, N(0,1): forward against the FP64 oracle, round
+    trip against the input.  Seed 24 (the workload's), and for FP32, whose forward error sits
+    near its 2e-7 bar, seeds 25 and 26 too; the worst seed is asserted."""
     n = 2 ** 24
-    x = inputs.complex_normal(n, 24).astype(dtype)
-    ref = oracle_fwd(x)
-    print(f"case 2^24 dtype={np.dtype(dtype).name}", flush=True)
+    seeds = (24,) if dtype == np.complex128 else (24, 25, 26)
+    worst_fwd = worst_rt = 0.0
     with api.Plan(n, VT[dtype], api.DFT, api.IN_PLACE) as pf, \
             api.Plan(n, VT[dtype], api.IDFT, api.IN_PLACE) as pi:
-        d = dev(torch_cuda, x)
-        pf.fft(d)
-        y = host(d)
-        e_fwd = rel_l2(y, ref)
-        pi.fft(d)
-        e_rt = rel_l2(host(d), x)
-    print(f"2^24 {np.dtype(dtype).name}: fwd relL2 {e_fwd:.3e} round-trip {e_rt:.3e}")
-    assert e_fwd <= TOL[dtype]
-    assert e_rt <= 2 * TOL[dtype]
+        for seed in seeds:
+            x = inputs.complex_normal(n, seed).astype(dtype)
+            ref = oracle_fwd(x)
+            print(f"case 2^24 dtype={np.dtype(dtype).name} seed={seed}", flush=True)
+            d = dev(torch_cuda, x)
+            pf.fft(d)
+            y = host(d)
+            e_fwd = rel_l2(y, ref)
+            pi.fft(d)
+            e_rt = rel_l2(host(d), x)
+            print(f"2^24 {np.dtype(dtype).name} seed {seed}: fwd relL2 {e_fwd:.3e} round-trip {e_rt:.3e}")
+            worst_fwd, worst_rt = max(worst_fwd, e_fwd), max(worst_rt, e_rt)
+    print(f"2^24 {np.dtype(dtype).name}: worst fwd relL2 {worst_fwd:.3e}, worst round trip {worst_rt:.3e}")
+    assert worst_fwd <= TOL[dtype]
+    assert worst_rt <= 2 * TOL[dtype]
 
 
 @pytest.mark.parametrize("dtype,log2n", [(np.complex64, 28), (np.complex128, 26)])

@@ -130,3 +130,38 @@ def test_plans_on_two_streams_match_serial(torch_cuda):
         torch.cuda.synchronize()
         assert torch.equal(ca, ra)
         assert torch.equal(cb, rb)
+
+
+@pytest.mark.parametrize("case", ["multipass_f32", "graph_f32", "ragged_f32"])
+def test_one_plan_on_two_streams_matches_serial(torch_cuda, case):
+    """Calls on ONE plan issued alternately on two unordered streams, each on its own buffer,
+    give bit for bit the serial results: the plan's scratch arrays serve every call, and a
+    call on the other stream waits for the previous call's work on the device."""
+    torch = torch_cuda
+    print(f"case {case}", flush=True)
+    if case == "ragged_f32":
+        lengths = [4096, 2 ** 15, 3 * 2 ** 14, 1009, 256, 2 ** 16]
+        tot = sum(lengths)
+        xs = [torch.from_numpy(inputs.complex_normal(tot, 95 + i).astype(np.complex64)).cuda() for i in range(2)]
+        plan = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT)
+    else:
+        n = 2 ** 22
+        xs = [torch.from_numpy(inputs.complex_normal(n, 95 + i).astype(np.complex64)).cuda() for i in range(2)]
+        flags = api.FLAG_GRAPH if case == "graph_f32" else 0
+        plan = api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.IN_PLACE, flags=flags)
+    refs = [x.clone() for x in xs]
+    for _ in range(2):
+        for r in refs:
+            plan.fft(r)
+    torch.cuda.synchronize()
+    bufs = [x.clone() for x in xs]
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    for _ in range(2):
+        for b, s in zip(bufs, streams):
+            with torch.cuda.stream(s):
+                plan.fft(b)
+    torch.cuda.synchronize()
+    for b, r in zip(bufs, refs):
+        assert torch.equal(b, r)
+    plan.close()
