@@ -65,6 +65,11 @@ def hbm_traffic(kernel_name):
     return int(round((fetch + write) * 1024))
 
 
+def log(msg):
+    """Progress line on stderr (the result line alone goes to stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def errors(y, ref):
     """Errors of a computed transform y against an FP64 reference: relL2 = |y-ref|_2/|ref|_2,
     max_abs = largest |re| or |im| error component, rel_inf = max_abs / (largest |re| or |im|
@@ -309,6 +314,7 @@ def other_workloads(torch, api):
             lat.append(time.perf_counter() - t0)
             q.fft(x1)
             torch.cuda.synchronize()
+    log("C1 done")
     res["c1_fp64_1024_single"] = {"us_per_call_back_to_back": round(us1, 2),
                                   "synchronous_call_us_median": round(float(np.median(lat)) * 1e6, 1),
                                   "errors_vs_numpy_fp64": e1}
@@ -328,6 +334,7 @@ def other_workloads(torch, api):
         us_graph = _call_time_us(torch, lambda: pg.fft(z, o), iters=200)
     with api.Plan(nr // 2, api.FLOAT, api.RDFT, api.OUT_OF_PLACE, flags=api.FLAG_R2C_BINS | api.FLAG_PROFILE) as p:
         ks = _kernel_time_us(p, lambda: p.fft(z, o))
+    log("C3 done")
     res["c3_r2c_3x2p20_f32"] = {"us": round(us, 1), "graph_us": round(us_graph, 1),
                                 "GFLOPs_2.5NlogN": round(2.5 * nr * np.log2(nr) / us / 1e3, 1),
                                 "GBps_alg": round((4 * nr + 8 * (nr // 2 + 1)) / us / 1e3, 1),
@@ -351,6 +358,7 @@ def other_workloads(torch, api):
     us_direct = _call_time_us(torch, lambda: rp.fft(xg), iters=30)
     rp.close()
     flops = sum(5 * L * np.log2(L) for L in lengths)
+    log("C4 done")
     res["c4_ragged_4096_f32"] = {"us": round(us, 1), "launch": "hipGraph replay", "direct_us": round(us_direct, 1),
                                  "GFLOPs": round(flops / us / 1e3, 1),
                                  "GBps_alg": round(2 * tot * 8 / us / 1e3, 1), "points": tot,
@@ -386,9 +394,12 @@ def other_workloads(torch, api):
     c5["mfma"]["mfma_flops_per_launch"] = mflops
     c5["mfma"]["mfma_TFLOPs"] = round(mflops / c5["mfma"]["us"] / 1e6, 1)
     c5["mfma"]["mfma_frac_of_fp16_peak"] = round(mflops / c5["mfma"]["us"] / 1e6 / 2500.0, 4)
+    log("C5 done")
     res["c5_16384x4096_f32"] = c5
     res["host_pointers"] = host_pointer_workloads(torch, api)
+    log("host pointers done")
     res["vs_torch_fft"] = vendor_fft_comparison(torch, api)
+    log("torch.fft comparison done")
     # any-N path: a prime length (chirp-z convolution of smooth length m >= 2n-1), FP32
     n = 1048573  # prime
     xp_h = inputs.complex_normal(n, 13).astype(np.complex64)
@@ -591,6 +602,7 @@ def main():
     # > 256 MB of rotating input buffers
     nbuf32 = max(2, math.ceil(512 * 2 ** 20 / (n * 8)))
     r32 = round_trip_run(torch, api, n, np.complex64, args.steps, args.warmup, nbuf32)
+    log("FP32 round trip done")
     flops_step = 2 * 5 * n * args.log2n
     ms_step = r32["ms"] / args.steps
     value = flops_step / (ms_step * 1e-3) / 1e9
@@ -673,6 +685,7 @@ def main():
         r64 = round_trip_run(torch, api, n, np.complex128, max(10, args.steps // 2),
                              args.warmup, nbuf64)
         ms64 = r64["ms"] / max(10, args.steps // 2)
+        log("FP64 round trip done")
         out["fp64"] = {"GFLOPs": round(flops_step / (ms64 * 1e-3) / 1e9, 2),
                        "ms_per_step": round(ms64, 4),
                        "errors": {"forward_vs_numpy_fp64": r64["fwd_err"], "round_trip": r64["rt_err"]},
