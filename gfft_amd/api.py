@@ -35,7 +35,7 @@ EXPORTED = [
     "gfftx_plan_create", "gfftx_plan_create_batched", "gfftx_exec_inplace", "gfftx_exec_oop",
     "gfftx_plan_destroy", "gfftx_plan_describe", "gfftx_plan_profile", "gfftx_ragged_create", "gfftx_ragged_exec",
     "gfftx_ragged_destroy", "gfftx_ragged_describe", "gfftx_ragged_profile", "gfftx_digit_reverse",
-    "gfftx_stream_synchronize", "gfftx_strerror", "gfftx_last_error",
+    "gfftx_stream_synchronize", "gfftx_strerror", "gfftx_last_error", "gfftx_exec_batch",
 ]
 
 _lib = None
@@ -78,6 +78,7 @@ def lib():
         L.gfftx_ragged_profile.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64)]
         L.gfftx_digit_reverse.argtypes = [vp, vp, u64, u32, ci, vp]
         L.gfftx_stream_synchronize.argtypes = [vp]
+        L.gfftx_exec_batch.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.c_size_t, vp]
         L.gfftx_strerror.argtypes = [ci]
         L.gfftx_strerror.restype = ctypes.c_char_p
         L.gfftx_last_error.restype = ctypes.c_char_p
@@ -273,6 +274,19 @@ class RaggedPlan:
             self.close()
         except Exception:
             pass
+
+
+def exec_batch(plans, buffers, stream=None):
+    """gfftx_exec_batch: plans[i] transforms buffers[i] in place, in order, in one call."""
+    if len(plans) != len(buffers):
+        raise ValueError("one buffer per plan")
+    for p, b in zip(plans, buffers):
+        _check_buffer(b, p.inplace_bytes, p.fp64, p.device, "exec_batch")
+    n = len(plans)
+    hs = (ctypes.c_void_p * n)(*[p._h.value if isinstance(p._h, ctypes.c_void_p) else p._h for p in plans])
+    ds = (ctypes.c_void_p * n)(*[_ptr(b) for b in buffers])
+    _check(lib().gfftx_exec_batch(hs, ds, n, _stream(buffers[0], stream) if n else None), "exec_batch")
+    return buffers
 
 
 def digit_reverse(src, dst, radix: int, digits: int, vtype: int, stream=None):

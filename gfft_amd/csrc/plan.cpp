@@ -659,6 +659,25 @@ int gfftx_exec_oop(gfftx_plan plan, const void* src, void* dst, void* stream)
     return exec_any(plan, src, dst, (hipStream_t)stream);
 }
 
+int gfftx_exec_batch(const gfftx_plan* plans, void* const* data, size_t count, void* stream)
+{
+    if (count == 0) return GFFTX_SUCCESS;
+    if (!plans || !data) return fail(GFFTX_ERR_INVALID_ARG, "plans/data is NULL");
+    for (size_t i = 0; i < count; ++i) {
+        if (!plans[i]) return fail(GFFTX_ERR_INVALID_ARG, "plans[%zu] is NULL", i);
+        if (!data[i]) return fail(GFFTX_ERR_INVALID_ARG, "data[%zu] is NULL", i);
+        if (plans[i]->place != GFFTX_IN_PLACE) return fail(GFFTX_ERR_INVALID_ARG, "plans[%zu] is out-of-place", i);
+    }
+    for (size_t i = 0; i < count; ++i) {
+        const int rc = exec_any(plans[i], data[i], data[i], (hipStream_t)stream);
+        if (rc) {
+            const std::string msg = g_last_error;
+            return fail(rc, "exec_batch item %zu: %s", i, msg.c_str());
+        }
+    }
+    return GFFTX_SUCCESS;
+}
+
 int gfftx_stream_synchronize(void* stream)
 {
     const hipError_t e = hipStreamSynchronize((hipStream_t)stream);

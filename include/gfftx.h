@@ -105,6 +105,17 @@ int gfftx_exec_inplace(gfftx_plan plan, void* data, void* stream);
  * AbstractFFT_oop<T>::fft (src/gfftpolicy.h:53-58). */
 int gfftx_exec_oop(gfftx_plan plan, const void* src, void* dst, void* stream);
 
+/* Several in-place transforms in one call: plans[i] transforms data[i], i < count, in that
+ * order on `stream` (device pointers: enqueued; host pointers: each returns when done, as
+ * gfftx_exec_inplace).  The same plan may appear several times.  Every plan must be an
+ * IN_PLACE plan and every pointer non-NULL; otherwise nothing is enqueued and
+ * GFFTX_ERR_INVALID_ARG is returned.  A failure part-way returns that call's status, with
+ * its index in gfftx_last_error(); the calls before it stay enqueued.  Replaces: a loop of
+ * fft(T* data) calls over GFFT objects, e.g. the disjoint-chunk loop of
+ * test/gfft_performance.h:110-116 or one object per length from one GenerateTransform
+ * factory (src/gfftgen.h:222-231). */
+int gfftx_exec_batch(const gfftx_plan* plans, void* const* data, size_t count, void* stream);
+
 /* Wait until the work enqueued on `stream` (a hipStream_t; NULL: the null stream) has
  * finished.  No reference counterpart: GFFT's fft() is synchronous
  * (src/gfftpolicy.h:163-172, :203-212), and the C++ policy layer calls this after each

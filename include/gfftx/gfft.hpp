@@ -1,18 +1,28 @@
 // gfft.hpp -- GFFT's plan-and-call surface with a GPU parallelisation policy HIP<Dev>.
 //
 // Host-only C++17 over the C ABI in gfftx.h; it needs no HIP headers and links against
-// libgfftx.so.  Names, IDs and defaults follow the reference:
+// libgfftx.so.  Names, IDs and defaults follow the reference, so that GFFT user code compiles
+// against this header (namespace GFFT is an alias of gfftx):
 //   value types   DOUBLE/FLOAT/COMPLEX_DOUBLE/COMPLEX_FLOAT      src/gfftpolicy.h:78-137
 //   transforms    DFT/IDFT/RDFT/IRDFT                            src/gfftpolicy.h:225-297
 //   places        IN_PLACE/OUT_OF_PLACE                          src/gfftpolicy.h:143-215
+//   parallel      Serial, OpenMP<NT>                             src/gfftpolicy.h:371-453
 //   interfaces    AbstractFFT_inp<T>::fft(T*),
 //                 AbstractFFT_oop<T>::fft(const T*, T*)          src/gfftpolicy.h:46-58
-//   Transform<N,VType,Type,Dim,Parall,Place>::Instance           src/gfftgen.h:46-87
+//   Transform<N,VType,Type,Dim,Parall,Place,IDN>::Instance, ::ID src/gfftgen.h:46-87
 //   GenerateTransform<NList,VType,Type,Dim,Parall,Place>
-//       ::CreateTransformObject(n, vtype, trans, dim, parall, place)  src/gfftgen.h:189-233
-//   group defaults DFT / OUT_OF_PLACE                            src/gfftparamgroups.h:44-78
-// HIP<Dev> replaces Serial/OpenMP<NT>: the transform runs on GPU `Dev`.  Its ID is
-// HIP_ID_BASE + Dev so that it never collides with OpenMP<NT>::ID = NT-1.
+//       ::CreateTransformObject(n, vtype, trans, dim, parall, place), ::Result
+//                                                                src/gfftgen.h:189-233
+//   groups        ValueTypeGroup, TransformTypeGroup,
+//                 ParallelizationGroup, PlaceGroup               src/gfftparamgroups.h:35-78
+//   length lists  TYPELIST_n, GenNumList, GenPowerList           loki/Typelist.h:33-,
+//                                                                src/typelistgen.h:101-147
+// Parameter lists are Loki typelists, as in the reference: when Loki's Typelist.h is not
+// included first, this header declares the subset GFFT's interface uses (Typelist,
+// NullType, TYPELIST_1..TYPELIST_8, TypeTraits<T>::isStdFundamental).
+// HIP<Dev> runs the transform on GPU `Dev`; its ID is HIP_ID_BASE + Dev so that it never
+// collides with OpenMP<NT>::ID = NT-1.  Serial and OpenMP<NT> keep their IDs and run on GPU 0:
+// the engine has no CPU path, and code written for them runs unchanged.
 // fft() is synchronous, as GFFT's is: it returns once the result is in `data`/`dst`
 // (host or device memory).
 #pragma once
@@ -24,19 +34,54 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
-#include <tuple>
 #include <type_traits>
 #include <utility>
 #include <vector>
 
 #include "../gfftx.h"
 
+// --- the part of Loki's interface GFFT's parameter lists use (loki/Typelist.h,
+// loki/NullType.h, loki/TypeTraits.h): skipped where Loki's own headers came first
+#if !defined(TYPELIST_INC_) && !defined(GFFTX_LOKI_TYPELIST)
+#define GFFTX_LOKI_TYPELIST
+namespace Loki {
+class NullType {};
+template <class T, class U> struct Typelist {
+    typedef T Head;
+    typedef U Tail;
+};
+}  // namespace Loki
+#define TYPELIST_1(T1) ::Loki::Typelist<T1, ::Loki::NullType>
+#define TYPELIST_2(T1, T2) ::Loki::Typelist<T1, TYPELIST_1(T2)>
+#define TYPELIST_3(T1, T2, T3) ::Loki::Typelist<T1, TYPELIST_2(T2, T3)>
+#define TYPELIST_4(T1, T2, T3, T4) ::Loki::Typelist<T1, TYPELIST_3(T2, T3, T4)>
+#define TYPELIST_5(T1, T2, T3, T4, T5) ::Loki::Typelist<T1, TYPELIST_4(T2, T3, T4, T5)>
+#define TYPELIST_6(T1, T2, T3, T4, T5, T6) ::Loki::Typelist<T1, TYPELIST_5(T2, T3, T4, T5, T6)>
+#define TYPELIST_7(T1, T2, T3, T4, T5, T6, T7) ::Loki::Typelist<T1, TYPELIST_6(T2, T3, T4, T5, T6, T7)>
+#define TYPELIST_8(T1, T2, T3, T4, T5, T6, T7, T8) ::Loki::Typelist<T1, TYPELIST_7(T2, T3, T4, T5, T6, T7, T8)>
+#endif
+#if !defined(TYPETRAITS_INC_) && !defined(GFFTX_LOKI_TYPETRAITS)
+#define GFFTX_LOKI_TYPETRAITS
+namespace Loki {
+template <typename T> struct TypeTraits {
+    enum { isStdFundamental = std::is_fundamental<T>::value };
+};
+}  // namespace Loki
+#endif
+
 namespace gfftx {
 
 typedef unsigned int id_t;
+typedef long long long_t;
+typedef unsigned long ulong_t;
 
-template <unsigned long N> struct ulong_ {
-    static const unsigned long value = N;
+// Length holders: value N, ID N-1 (src/typelistgen.h:32-35)
+template <ulong_t N> struct ulong_ {
+    static const ulong_t value = N;
+    static const id_t ID = (id_t)(N - 1);
+};
+template <long_t N> struct long_ {
+    static const long_t value = N;
     static const id_t ID = (id_t)(N - 1);
 };
 
@@ -120,8 +165,49 @@ static const id_t HIP_ID_BASE = 64;
 // GPU parallelisation policy: the transform runs on HIP device Dev.
 template <int Dev = 0> struct HIP {
     static const id_t ID = HIP_ID_BASE + (id_t)Dev;
+    static const ulong_t NParProc = 1;
     static const int Device = Dev;
     static const char* name() { return "hip"; }
+};
+
+// GFFT's CPU policies, kept with their IDs (Serial 0, OpenMP<NT> NT-1; OpenMP<0> and
+// OpenMP<1> are Serial, src/gfftpolicy.h:371-453).  Here they run on GPU 0.
+struct Serial {
+    static const id_t ID = 0;
+    static const ulong_t NParProc = 1;
+    static const int Device = 0;
+    static const char* name() { return "serial (GPU 0)"; }
+};
+template <unsigned int NT> struct OpenMP {
+    static const id_t ID = NT - 1;
+    static const ulong_t NParProc = NT;
+    static const int Device = 0;
+    static const char* name() { return "openmp (GPU 0)"; }
+};
+template <> struct OpenMP<0> : public Serial {};
+template <> struct OpenMP<1> : public Serial {};
+
+// Parameter groups (src/gfftparamgroups.h:35-78): FullList, Length and Default as the
+// reference states them (ParallelizationGroup's Length reads 3 there for its four entries).
+struct ValueTypeGroup {
+    typedef TYPELIST_4(DOUBLE, FLOAT, COMPLEX_DOUBLE, COMPLEX_FLOAT) FullList;
+    static const ulong_t Length = 4;
+    typedef DOUBLE Default;
+};
+struct TransformTypeGroup {
+    typedef TYPELIST_4(DFT, IDFT, RDFT, IRDFT) FullList;
+    static const ulong_t Length = 4;
+    typedef DFT Default;
+};
+struct ParallelizationGroup {
+    typedef TYPELIST_4(OpenMP<1>, OpenMP<2>, OpenMP<4>, OpenMP<8>) FullList;
+    static const ulong_t Length = 3;
+    typedef Serial Default;
+};
+struct PlaceGroup {
+    typedef TYPELIST_2(IN_PLACE, OUT_OF_PLACE) FullList;
+    static const ulong_t Length = 2;
+    typedef OUT_OF_PLACE Default;
 };
 
 // Error type of an unknown combination, as TransformFactoryError (src/gfftgen.h:133-145).
@@ -180,162 +266,8 @@ private:
     gfftx_plan plan_ = nullptr;
 };
 
-template <class VType, class Type, class Place, class Parall> struct Exec;
-
-template <class VType, class Type, class Parall>
-struct Exec<VType, Type, IN_PLACE, Parall> : public AbstractFFT_inp<typename VType::ValueType> {
-    typedef typename VType::ValueType T;
-    PlanHolder h;
-    explicit Exec(unsigned long n) : h(n, VType::ID, Type::ID, IN_PLACE::ID, Parall::Device) {}
-    void fft(T* data) override
-    {
-        check(gfftx_exec_inplace(h.get(), data, nullptr), "fft(T*)");
-        check(gfftx_stream_synchronize(nullptr), "fft(T*)");
-    }
-};
-
-template <class VType, class Type, class Parall>
-struct Exec<VType, Type, OUT_OF_PLACE, Parall> : public AbstractFFT_oop<typename VType::ValueType> {
-    typedef typename VType::ValueType T;
-    PlanHolder h;
-    explicit Exec(unsigned long n) : h(n, VType::ID, Type::ID, OUT_OF_PLACE::ID, Parall::Device) {}
-    void fft(const T* src, T* dst) override
-    {
-        check(gfftx_exec_oop(h.get(), src, dst, nullptr), "fft(const T*, T*)");
-        check(gfftx_stream_synchronize(nullptr), "fft(const T*, T*)");
-    }
-};
-
-}  // namespace detail
-
-// Transform<N, VType, Type, Dim, Parall, Place>: one fixed transform (src/gfftgen.h:46-87).
-// N is ulong_<n>; n is the complex length (half the real length for RDFT/IRDFT).
-template <class N, class VType, class Type = DFT, class Dim = ulong_<1>, class Parall = HIP<0>,
-          class Place = OUT_OF_PLACE>
-class Transform {
-    static_assert(Dim::value == 1, "only one-dimensional transforms exist (src/gfftgen.h:154)");
-
-public:
-    typedef VType ValueType;
-    typedef Type TransformType;
-    typedef Parall ParallType;
-    typedef Place PlaceType;
-    static const unsigned long Len = N::value;
-    typedef typename Place::template Interface<typename VType::ValueType>::Result ReturnType;
-
-    class Instance : public detail::Exec<VType, Type, Place, Parall> {
-    public:
-        Instance() : detail::Exec<VType, Type, Place, Parall>(N::value) {}
-    };
-
-    static ReturnType* Create() { return new Instance(); }
-};
-
-// Parameter lists (the typelists of src/gfftparamgroups.h as variadic packs).
-template <class... Ts> struct List {};
-template <unsigned long... Ns> struct NList {};
-
-// GenPowerList<Min, Max, Base>: lengths Base^Min .. Base^Max (src/typelistgen.h:116-147).
-namespace detail {
-template <unsigned long B, unsigned long P> struct Pow {
-    static const unsigned long value = B * Pow<B, P - 1>::value;
-};
-template <unsigned long B> struct Pow<B, 0> {
-    static const unsigned long value = 1;
-};
-template <unsigned long Min, unsigned long Max, unsigned long B, class Seq> struct PowSeq;
-template <unsigned long Min, unsigned long Max, unsigned long B, std::size_t... I>
-struct PowSeq<Min, Max, B, std::index_sequence<I...>> {
-    typedef NList<Pow<B, Min + I>::value...> type;
-};
-}  // namespace detail
-template <unsigned long Min, unsigned long Max, unsigned long Base = 2>
-using GenPowerList = typename detail::PowSeq<Min, Max, Base, std::make_index_sequence<Max - Min + 1>>::type;
-
-namespace detail {
-template <class X> struct AsList {
-    typedef List<X> type;
-};
-template <class... Ts> struct AsList<List<Ts...>> {
-    typedef List<Ts...> type;
-};
-}  // namespace detail
-
-// GenerateTransform: registers every combination of the parameter lists and creates
-// objects by run-time IDs (src/gfftgen.h:189-233).  TransType/Parall/Place may be single
-// classes or List<...>; all places must share one interface, as in the reference.
-template <class NL, class VType, class TransType = DFT, class Dim = ulong_<1>, class Parall = HIP<0>,
-          class Place = OUT_OF_PLACE>
-class GenerateTransform;
-
-template <unsigned long... Ns, class VType, class TransType, class Dim, class Parall, class Place>
-class GenerateTransform<NList<Ns...>, VType, TransType, Dim, Parall, Place> {
-    struct Key {
-        unsigned long n;
-        id_t vtype, trans, dim, parall, place;
-    };
-    std::vector<Key> keys_;
-
-    template <class T, class P, class Pl> void add_one()
-    {
-        for (unsigned long n : {Ns...}) keys_.push_back(Key{n, VType::ID, T::ID, (id_t)Dim::value, P::ID, Pl::ID});
-    }
-    template <class T, class P, class... Pls> void add_places(List<Pls...>)
-    {
-        (add_one<T, P, Pls>(), ...);
-    }
-    template <class T, class... Ps> void add_paralls(List<Ps...>)
-    {
-        (add_places<T, Ps>(typename detail::AsList<Place>::type()), ...);
-    }
-    template <class... Ts> void add_types(List<Ts...>)
-    {
-        (add_paralls<Ts>(typename detail::AsList<Parall>::type()), ...);
-    }
-    template <class X> struct FirstOf {
-        typedef X type;
-    };
-    template <class X, class... Rest> struct FirstOf<List<X, Rest...>> {
-        typedef X type;
-    };
-    template <class P, class... Ps> static int device_of(id_t parall_id, List<P, Ps...>)
-    {
-        if (P::ID == parall_id) return P::Device;
-        if constexpr (sizeof...(Ps) > 0) return device_of(parall_id, List<Ps...>());
-        return -1;
-    }
-
-public:
-    typedef typename FirstOf<Place>::type PlaceType;
-    typedef typename PlaceType::template Interface<typename VType::ValueType>::Result ObjectType;
-
-    GenerateTransform() { add_types(typename detail::AsList<TransType>::type()); }
-
-    ObjectType* CreateTransformObject(unsigned long n, unsigned long vtype_id, unsigned long trans_id = DFT::ID,
-                                      unsigned long dim = 1,
-                                      unsigned long parall_id = FirstOf<Parall>::type::ID,
-                                      unsigned long place_id = PlaceType::ID)
-    {
-        bool found = false;
-        for (const Key& k : keys_)
-            found = found || (k.n == n && k.vtype == vtype_id && k.trans == trans_id && k.dim == dim &&
-                              k.parall == parall_id && k.place == place_id);
-        if (!found) throw TransformFactoryError::Exception();
-        const int dev = device_of((id_t)parall_id, typename detail::AsList<Parall>::type());
-        return make(n, (id_t)trans_id, dev);
-    }
-
-private:
-    ObjectType* make(unsigned long n, id_t trans, int dev)
-    {
-        // The object type is fixed by the interface; the transform type is a run-time ID.
-        return new RuntimeExec(n, trans, dev);
-    }
-
-    class RuntimeExec;
-};
-
-namespace detail {
+// The transform object for a place: the interface GFFT's place policy names, with fft()
+// calling the C ABI and waiting for the result.
 template <class VType, class Place> class RuntimeObject;
 template <class VType>
 class RuntimeObject<VType, IN_PLACE> : public AbstractFFT_inp<typename VType::ValueType> {
@@ -363,17 +295,190 @@ public:
 private:
     PlanHolder h_;
 };
+
+// --- typelist helpers: a single class counts as a one-element list
+template <class X> struct AsList {
+    typedef Loki::Typelist<X, Loki::NullType> type;
+};
+template <class H, class T> struct AsList<Loki::Typelist<H, T>> {
+    typedef Loki::Typelist<H, T> type;
+};
+template <> struct AsList<Loki::NullType> {
+    typedef Loki::NullType type;
+};
+template <class A, class B> struct Append;
+template <class B> struct Append<Loki::NullType, B> {
+    typedef B type;
+};
+template <class H, class T, class B> struct Append<Loki::Typelist<H, T>, B> {
+    typedef Loki::Typelist<H, typename Append<T, B>::type> type;
+};
+template <class L> struct Head {
+    typedef typename AsList<L>::type::Head type;
+};
+// every place in the list has the first one's interface
+template <class L, class P> struct SameInterface {
+    static const bool value = true;
+};
+template <class H, class T, class P> struct SameInterface<Loki::Typelist<H, T>, P> {
+    static const bool value = std::is_same<typename H::template Interface<int>::Result,
+                                           typename P::template Interface<int>::Result>::value &&
+                              SameInterface<T, P>::value;
+};
+
 }  // namespace detail
 
-template <unsigned long... Ns, class VType, class TransType, class Dim, class Parall, class Place>
-class GenerateTransform<NList<Ns...>, VType, TransType, Dim, Parall, Place>::RuntimeExec
-    : public detail::RuntimeObject<VType, typename GenerateTransform<NList<Ns...>, VType, TransType, Dim, Parall,
-                                                                       Place>::PlaceType> {
+// Transform<N, VType, Type, Dim, Parall, Place, IDN>: one fixed transform (src/gfftgen.h:46-87).
+// N is a length holder (ulong_<n>); n is the complex length (half the real length for
+// RDFT/IRDFT).  IDN is the factory ID GFFT gives the combination (N::ID by default).
+template <class N, class VType, class Type = DFT, class Dim = ulong_<1>, class Parall = HIP<0>,
+          class Place = OUT_OF_PLACE, id_t IDN = N::ID>
+class Transform {
+    static_assert(Dim::value == 1, "only one-dimensional transforms exist (src/gfftgen.h:154)");
+
 public:
-    RuntimeExec(unsigned long n, id_t trans, int dev)
-        : detail::RuntimeObject<VType, typename GenerateTransform::PlaceType>(n, trans, dev)
+    typedef VType ValueType;
+    typedef Type TransformType;
+    typedef Parall ParallType;
+    typedef Place PlaceType;
+    typedef typename Place::template Interface<typename VType::ValueType>::Result ReturnType;
+
+    enum { ID = IDN };
+    static const long_t Len = (long_t)N::value;
+
+    class Instance : public detail::RuntimeObject<VType, Place> {
+    public:
+        Instance() : detail::RuntimeObject<VType, Place>((unsigned long)N::value, Type::ID, Parall::Device) {}
+    };
+
+    static ReturnType* Create() { return new Instance(); }
+};
+
+// Typelist of holders N = Begin..End (src/typelistgen.h:101-114)
+template <long_t Begin, long_t End, template <long_t> class Holder = long_> struct GenNumList {
+    typedef Loki::Typelist<Holder<Begin>, typename GenNumList<Begin + 1, End, Holder>::Result> Result;
+};
+template <long_t End, template <long_t> class Holder> struct GenNumList<End, End, Holder> {
+    typedef Loki::Typelist<Holder<End>, Loki::NullType> Result;
+};
+
+// Length M^P with value and ID = value - 1 (src/typelistgen.h:117-131)
+template <long_t M, long_t P> struct PowerHolder {
+    static const long_t N = M * PowerHolder<M, P - 1>::N;
+    static const long_t ID = N - 1;
+    static const long_t value = N;
+};
+template <long_t M> struct PowerHolder<M, 0> {
+    static const long_t N = 1;
+    static const long_t ID = 0;
+    static const long_t value = 1;
+};
+
+// Typelist of PowerHolder<M, P>, P = PowerBegin..PowerEnd (src/typelistgen.h:137-147)
+template <long_t PowerBegin, long_t PowerEnd, long_t M> struct GenPowerList {
+    typedef Loki::Typelist<PowerHolder<M, PowerBegin>, typename GenPowerList<PowerBegin + 1, PowerEnd, M>::Result>
+        Result;
+};
+template <long_t PowerEnd, long_t M> struct GenPowerList<PowerEnd, PowerEnd, M> {
+    typedef Loki::Typelist<PowerHolder<M, PowerEnd>, Loki::NullType> Result;
+};
+
+namespace detail {
+
+// Every combination of the parameter lists as Transform classes: lengths outermost, then
+// transform types, parallelisations and places.
+template <class N, class V, class T, class Dim, class P, class PlL> struct ForPlaces {
+    typedef Loki::NullType type;
+};
+template <class N, class V, class T, class Dim, class P, class Pl, class R>
+struct ForPlaces<N, V, T, Dim, P, Loki::Typelist<Pl, R>> {
+    typedef Loki::Typelist<Transform<N, V, T, Dim, P, Pl>, typename ForPlaces<N, V, T, Dim, P, R>::type> type;
+};
+template <class N, class V, class T, class Dim, class PL, class PlL> struct ForParalls {
+    typedef Loki::NullType type;
+};
+template <class N, class V, class T, class Dim, class P, class R, class PlL>
+struct ForParalls<N, V, T, Dim, Loki::Typelist<P, R>, PlL> {
+    typedef typename Append<typename ForPlaces<N, V, T, Dim, P, PlL>::type,
+                            typename ForParalls<N, V, T, Dim, R, PlL>::type>::type type;
+};
+template <class N, class V, class TL, class Dim, class PL, class PlL> struct ForTypes {
+    typedef Loki::NullType type;
+};
+template <class N, class V, class T, class R, class Dim, class PL, class PlL>
+struct ForTypes<N, V, Loki::Typelist<T, R>, Dim, PL, PlL> {
+    typedef typename Append<typename ForParalls<N, V, T, Dim, PL, PlL>::type,
+                            typename ForTypes<N, V, R, Dim, PL, PlL>::type>::type type;
+};
+template <class NL, class V, class TL, class Dim, class PL, class PlL> struct Product {
+    typedef Loki::NullType type;
+};
+template <class N, class R, class V, class TL, class Dim, class PL, class PlL>
+struct Product<Loki::Typelist<N, R>, V, TL, Dim, PL, PlL> {
+    typedef typename Append<typename ForTypes<N, V, TL, Dim, PL, PlL>::type,
+                            typename Product<R, V, TL, Dim, PL, PlL>::type>::type type;
+};
+
+// Run-time keys of every combination in a list of Transform classes.
+struct Key {
+    unsigned long n;
+    id_t vtype, trans, dim, parall, place;
+    int device;
+};
+template <class L> struct Keys {
+    static void add(std::vector<Key>&) {}
+};
+template <class H, class T> struct Keys<Loki::Typelist<H, T>> {
+    static void add(std::vector<Key>& v)
     {
+        v.push_back(Key{(unsigned long)H::Len, H::ValueType::ID, H::TransformType::ID, 1u, H::ParallType::ID,
+                        H::PlaceType::ID, H::ParallType::Device});
+        Keys<T>::add(v);
+    }
+};
+
+}  // namespace detail
+
+// GenerateTransform: registers every combination of the parameter lists and creates objects
+// by run-time IDs (src/gfftgen.h:189-233).  NList is a typelist of length holders;
+// TransType, Parall and Place may be single classes or typelists; all places share one
+// interface, as in the reference.  Result is the typelist of the Transform classes.
+template <class NList, class VType, class TransType = TransformTypeGroup::Default, class Dim = ulong_<1>,
+          class Parall = ParallelizationGroup::Default, class Place = PlaceGroup::Default>
+class GenerateTransform {
+    typedef typename detail::AsList<NList>::type NL;
+    typedef typename detail::AsList<TransType>::type TL;
+    typedef typename detail::AsList<Parall>::type PL;
+    typedef typename detail::AsList<Place>::type PlL;
+    std::vector<detail::Key> keys_;
+
+public:
+    typedef typename detail::Product<NL, VType, TL, Dim, PL, PlL>::type Result;
+    typedef typename detail::Head<Place>::type PlaceType;
+    typedef typename PlaceType::template Interface<typename VType::ValueType>::Result ObjectType;
+    static_assert(detail::SameInterface<PlL, PlaceType>::value,
+                  "the places of one GenerateTransform must share one interface (src/gfftgen.h:203)");
+
+    GenerateTransform() { detail::Keys<Result>::add(keys_); }
+
+    // The defaults for parall_id and place_id are the first entries of the Parall and Place
+    // lists (the reference takes the group defaults, Serial and OUT_OF_PLACE, which coincide
+    // when the lists are left at their defaults).
+    ObjectType* CreateTransformObject(ulong_t n, ulong_t vtype_id, ulong_t trans_id = TransformTypeGroup::Default::ID,
+                                      ulong_t dim = 1,
+                                      ulong_t parall_id = detail::Head<Parall>::type::ID,
+                                      ulong_t place_id = PlaceType::ID)
+    {
+        for (const detail::Key& k : keys_)
+            if (k.n == n && k.vtype == vtype_id && k.trans == trans_id && k.dim == dim && k.parall == parall_id &&
+                k.place == place_id)
+                return new detail::RuntimeObject<VType, PlaceType>(n, (id_t)trans_id, k.device);
+        throw TransformFactoryError::Exception();
     }
 };
 
 }  // namespace gfftx
+
+// GFFT's namespace: user code that names GFFT:: or says `using namespace GFFT;` compiles
+// against this header unchanged.
+namespace GFFT = gfftx;

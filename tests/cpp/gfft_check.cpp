@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <type_traits>
 #include <vector>
 
 #include "gfftx/gfft.hpp"
@@ -86,6 +87,74 @@ template <class VType, class Type, class Place, unsigned long N> static bool che
     }
 }
 
+// The sample program's pattern with GFFT's seq input x_m = (2m, 2m+1) (src/direct.h:52-56),
+// whose transform has the closed form X_0 = (2+2i) n(n-1)/2 + i n and
+// X_k = (2+2i) n i e^{i pi k/n} / (2 sin(pi k/n)) for k > 0.
+template <class ValueType, class Place, class Parall> static bool sample_program(const char* what, double bound)
+{
+    static const long_t N = 1024;
+    typedef TYPELIST_1(ulong_<N>) NList;
+    typedef GenerateTransform<NList, ValueType, TransformTypeGroup::FullList, ulong_<1>, Parall, Place> TransformSet;
+    typedef DFT TransformType;
+    typedef typename ValueType::ValueType T;
+    typedef typename ValueType::base_type BT;
+    static const int C = Loki::TypeTraits<T>::isStdFundamental ? 2 : 1;
+
+    TransformSet gfft;
+    typename TransformSet::ObjectType* fftobj =
+        gfft.CreateTransformObject(N, ValueType::ID, TransformType::ID, 1, Parall::ID, Place::ID);
+    std::vector<T> data(C * N), dataout(C * N);
+    BT* d = reinterpret_cast<BT*>(data.data());
+    for (long_t m = 0; m < N; ++m) {
+        d[2 * m] = (BT)(2 * m);
+        d[2 * m + 1] = (BT)(2 * m + 1);
+    }
+    BT* out = d;
+    if constexpr (std::is_same<Place, OUT_OF_PLACE>::value) {
+        fftobj->fft(data.data(), dataout.data());
+        out = reinterpret_cast<BT*>(dataout.data());
+    } else {
+        fftobj->fft(data.data());
+    }
+    delete fftobj;
+    const long double PI = 3.141592653589793238462643383279502884L;
+    std::vector<double> ref(2 * N);
+    ref[0] = 2.0 * N * (N - 1) / 2;
+    ref[1] = 2.0 * N * (N - 1) / 2 + N;
+    for (long_t k = 1; k < N; ++k) {
+        const long double a = PI * k / N, den = 2 * sinl(a), r = (long double)N / den;
+        // (2+2i) i e^{ia} n / (2 sin a) = r (2i - 2)(cos a + i sin a), r = n / (2 sin a)
+        ref[2 * k] = (double)(r * (-2 * cosl(a) - 2 * sinl(a)));
+        ref[2 * k + 1] = (double)(r * (2 * cosl(a) - 2 * sinl(a)));
+    }
+    return report(what, (long)N, out, ref, bound);
+}
+
+// The groups' lists, defaults and lengths as the reference states them
+// (src/gfftparamgroups.h:35-78), and the IDs of the parallelisation policies
+// (src/gfftpolicy.h:371-453, src/gfftgen.h:78).
+static bool groups_match_reference()
+{
+    bool ok = true;
+    ok &= std::is_same<ValueTypeGroup::Default, DOUBLE>::value && ValueTypeGroup::Length == 4;
+    ok &= std::is_same<ValueTypeGroup::FullList::Tail::Tail::Tail::Head, COMPLEX_FLOAT>::value;
+    ok &= std::is_same<TransformTypeGroup::Default, DFT>::value && TransformTypeGroup::Length == 4;
+    ok &= std::is_same<TransformTypeGroup::FullList::Tail::Tail::Head, RDFT>::value;
+    ok &= std::is_same<ParallelizationGroup::Default, Serial>::value && ParallelizationGroup::Length == 3;
+    ok &= std::is_same<ParallelizationGroup::FullList::Tail::Tail::Tail::Head, OpenMP<8>>::value;
+    ok &= std::is_same<PlaceGroup::Default, OUT_OF_PLACE>::value && PlaceGroup::Length == 2;
+    ok &= Serial::ID == 0 && OpenMP<1>::ID == 0 && OpenMP<4>::ID == 3 && OpenMP<8>::NParProc == 8;
+    typedef Transform<ulong_<256>, FLOAT, DFT, ulong_<1>, Serial, IN_PLACE> T256;
+    typedef Transform<ulong_<256>, FLOAT, DFT, ulong_<1>, Serial, IN_PLACE, 77> T256id;
+    ok &= (int)T256::ID == 255 && (int)T256id::ID == 77 && T256::Len == 256;
+    typedef GenPowerList<3, 5, 2>::Result P;
+    ok &= P::Head::value == 8 && P::Tail::Tail::Head::value == 32 && P::Head::ID == 7;
+    typedef GenNumList<20, 22>::Result Q;
+    ok &= Q::Tail::Head::value == 21;
+    std::printf("parameter groups and IDs %s\n", ok ? "ok" : "FAIL");
+    return ok;
+}
+
 int main()
 {
     bool ok = true;
@@ -102,7 +171,9 @@ int main()
     ok &= check_one<COMPLEX_FLOAT, IDFT, IN_PLACE, 1001>("COMPLEX_FLOAT IDFT 7*11*13", 5e-7);
 
     // Factory path: objects created by run-time IDs, as src/gfft.cpp:65-67 does.
-    typedef GenerateTransform<GenPowerList<4, 10>, DOUBLE, List<DFT, IDFT>, ulong_<1>, HIP<0>, OUT_OF_PLACE> Set;
+    typedef GenerateTransform<GenPowerList<4, 10, 2>::Result, DOUBLE, TYPELIST_2(DFT, IDFT), ulong_<1>, HIP<0>,
+                              OUT_OF_PLACE>
+        Set;
     Set gfft;
     for (unsigned long n : {16ul, 256ul, 1024ul}) {
         AbstractFFT_oop<double>* obj = gfft.CreateTransformObject(n, DOUBLE::ID, DFT::ID, 1, HIP<0>::ID,
@@ -122,6 +193,15 @@ int main()
         std::printf("unknown transform -> %s\n", e.what());
     }
     ok &= threw;
+
+    // GFFT's sample program (src/gfft.cpp:41-67) in the reference's own forms: a one-length
+    // TYPELIST, the full transform-type group, the value type's ValueType/base_type and
+    // Loki::TypeTraits, and CreateTransformObject with the policy's ID, for HIP<0> and for
+    // an unchanged OpenMP<4> (which runs on GPU 0 here), in place and out of place.
+    ok &= sample_program<DOUBLE, OUT_OF_PLACE, HIP<0>>("sample DOUBLE oop HIP<0>", 1e-13);
+    ok &= sample_program<DOUBLE, IN_PLACE, OpenMP<4>>("sample DOUBLE in-place OpenMP<4>", 1e-13);
+    ok &= sample_program<COMPLEX_FLOAT, OUT_OF_PLACE, Serial>("sample COMPLEX_FLOAT oop Serial", 2e-7);
+    ok &= groups_match_reference();
     std::printf("%s\n", ok ? "ALL OK" : "FAILURES");
     return ok ? 0 : 1;
 }

@@ -129,3 +129,15 @@ def test_cpp_policy_header_compiles_cleanly():
                         os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "cpp", "gfft_check.cpp")],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_exec_batch_argument_errors():
+    L = api.lib()
+    vp = ctypes.c_void_p
+    # an empty batch is a no-op; NULL arrays and NULL entries are rejected before any work
+    assert L.gfftx_exec_batch(None, None, 0, None) == api.SUCCESS
+    assert L.gfftx_exec_batch(None, None, 2, None) == api.ERR_INVALID_ARG
+    plans = (vp * 2)(None, None)
+    data = (vp * 2)(None, None)
+    assert L.gfftx_exec_batch(plans, data, 2, None) == api.ERR_INVALID_ARG
+    assert b"plans[0]" in L.gfftx_last_error()

@@ -157,3 +157,30 @@ def test_threads_share_one_plan(torch_cuda):
         for k in range(3):
             assert np.array_equal(got[i][k], alone[i]), (i, k)
             assert np.array_equal(got_r[i][k], ragged_alone[i]), (i, k)
+
+
+def test_exec_batch_equals_single_calls(torch_cuda):
+    """gfftx_exec_batch runs plans[i] on data[i] in order: the same plan twice, a batched FP64
+    plan and a multi-pass FP32 plan give bit for bit what separate calls give; an
+    out-of-place plan in the list is refused before anything runs."""
+    torch = torch_cuda
+    xs = [inputs.complex_normal(4096, 60).astype(np.complex64), inputs.complex_normal(4096, 61).astype(np.complex64),
+          inputs.complex_normal(1024 * 8, 62), inputs.complex_normal(2 ** 20, 63).astype(np.complex64)]
+    with api.Plan(4096, api.COMPLEX_FLOAT, api.DFT, api.IN_PLACE) as a, \
+            api.Plan(1024, api.COMPLEX_DOUBLE, api.IDFT, api.IN_PLACE, batch=8) as b, \
+            api.Plan(2 ** 20, api.COMPLEX_FLOAT, api.DFT, api.IN_PLACE) as c, \
+            api.Plan(4096, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE) as oop:
+        plans = [a, a, b, c]
+        refs = [torch.from_numpy(x).cuda() for x in xs]
+        for p, r in zip(plans, refs):
+            p.fft(r)
+        bufs = [torch.from_numpy(x).cuda() for x in xs]
+        api.exec_batch(plans, bufs)
+        torch.cuda.synchronize()
+        for r, o in zip(refs, bufs):
+            assert torch.equal(r, o)
+        keep = [x.clone() for x in bufs]
+        with pytest.raises(api.GfftxError):
+            api.exec_batch([a, oop], bufs[:2])
+        torch.cuda.synchronize()
+        assert torch.equal(keep[0], bufs[0])  # nothing ran
