@@ -259,7 +259,8 @@ def test_graph_cache_rotating_buffers(torch_cuda):
 
 def test_autotuner_measures_then_caches(torch_cuda, tmp_path, monkeypatch):
     """A multi-pass plan times several pass splits on the device, records the winner in the
-    JSON cache, and the next plan of the same shape reuses it."""
+    JSON cache, and the next plan of the same shape and place reuses it (forward and inverse
+    share an entry); an out-of-place plan of that length is tuned and cached on its own."""
     import json
 
     cache = tmp_path / "tune.json"
@@ -275,13 +276,21 @@ def test_autotuner_measures_then_caches(torch_cuda, tmp_path, monkeypatch):
     assert isinstance(d1["tuned"], list) and len(d1["tuned"]) >= 2
     text = cache.read_text()
     assert '"version": 2' in text and "n=1048576" in text
-    with api.Plan(n, api.COMPLEX_FLOAT, api.IDFT, api.OUT_OF_PLACE) as p:
+    with api.Plan(n, api.COMPLEX_FLOAT, api.IDFT, api.IN_PLACE) as p:
         d2 = json.loads(p.describe())
         assert d2["tuned"] == "cache"
+        s = dev(torch_cuda, ref.astype(np.complex64))
+        p.fft(s)
+        assert rel_l2(host(s), x) <= 4e-7
+    with api.Plan(n, api.COMPLEX_FLOAT, api.IDFT, api.OUT_OF_PLACE) as p:
+        d3 = json.loads(p.describe())
+        assert isinstance(d3["tuned"], list)
         s = dev(torch_cuda, ref.astype(np.complex64))
         o = torch_cuda.empty_like(s)
         p.fft(s, o)
         assert rel_l2(host(o), x) <= 4e-7
+    text = cache.read_text()
+    assert "n=1048576|b=1|ip" in text and "n=1048576|b=1|oop" in text
     with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.IN_PLACE, flags=api.FLAG_NO_AUTOTUNE) as p:
         assert "tuned" not in json.loads(p.describe())
 
