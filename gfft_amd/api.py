@@ -36,6 +36,7 @@ EXPORTED = [
     "gfftx_plan_destroy", "gfftx_plan_describe", "gfftx_plan_profile", "gfftx_ragged_create", "gfftx_ragged_exec",
     "gfftx_ragged_destroy", "gfftx_ragged_describe", "gfftx_ragged_profile", "gfftx_digit_reverse",
     "gfftx_stream_synchronize", "gfftx_strerror", "gfftx_last_error", "gfftx_exec_batch",
+    "gfftx_exec_inplace_noalloc",
 ]
 
 _lib = None
@@ -79,6 +80,7 @@ def lib():
         L.gfftx_digit_reverse.argtypes = [vp, vp, u64, u32, ci, vp]
         L.gfftx_stream_synchronize.argtypes = [vp]
         L.gfftx_exec_batch.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.c_size_t, vp]
+        L.gfftx_exec_inplace_noalloc.argtypes = [vp, vp, vp]
         L.gfftx_strerror.argtypes = [ci]
         L.gfftx_strerror.restype = ctypes.c_char_p
         L.gfftx_last_error.restype = ctypes.c_char_p

@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgfftx.so")
 
 SOURCES = ["kernels.hip", "ragged.hip", "bluestein.hip", "planner.cpp", "plan.cpp", "host_pipeline.cpp",
-           "tuner.cpp", "ragged_plan.cpp"]
+           "tuner.cpp", "ragged_plan.cpp", "noalloc.cpp"]
 # per-source flags: the MFMA kernels keep their products in VGPRs (no accumulator copies)
 EXTRA = {"kernels.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

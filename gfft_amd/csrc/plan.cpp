@@ -206,7 +206,7 @@ bool is_device_ptr(const void* ptr)
 bool prof_begin(gfftx_plan_s* p, size_t slot, const char* name, hipStream_t s, std::pair<hipEvent_t, hipEvent_t>* ev,
                 bool attached)
 {
-    if (!(p->flags & GFFTX_FLAG_PROFILE) || p->capturing) return false;
+    if (!(p->flags & GFFTX_FLAG_PROFILE) || p->capturing || p->no_events) return false;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;  // a capture the caller runs
     if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
         (void)hipGetLastError();
@@ -249,12 +249,15 @@ int run_passes(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, Range 
     const size_t m = p->passes.size();
     // buffer for the output of pass i (i < m-1): alternate so that no pass is in place and
     // the caller's input is never written unless it is also the output
-    std::vector<void*> dst(m);
+    void* const s0 = p->call_scratch[0] ? p->call_scratch[0] : p->scratch[0];
+    void* const s1 = p->call_scratch[0] ? p->call_scratch[1] : p->scratch[1];
+    void* dst[32];
+    if (m == 0 || m > 32) return fail(GFFTX_ERR_INVALID_ARG, "plan with %zu passes", m);
     dst[m - 1] = out;
     for (size_t i = m - 1; i-- > 0;) {
         void* next_src_dst = dst[i + 1];
-        void* cand = (next_src_dst == p->scratch[0]) ? out : p->scratch[0];
-        if (cand == in || cand == next_src_dst) cand = p->scratch[1];
+        void* cand = (next_src_dst == s0) ? out : s0;
+        if (cand == in || cand == next_src_dst) cand = s1;
         dst[i] = cand;
     }
     const void* src = in;

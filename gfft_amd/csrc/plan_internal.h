@@ -269,6 +269,10 @@ struct gfftx_plan_s {
     };
     std::vector<Slot> slots;
     bool capturing = false;
+    // a no-allocation call (gfftx_exec_inplace_noalloc) in progress: its passes use these
+    // intermediate arrays instead of `scratch`, and no profiling events are created
+    void* call_scratch[2] = {nullptr, nullptr};
+    bool no_events = false;
     // GFFTX_FLAG_GRAPH: launches captured once per (in, out) pair and replayed
     GraphCache graphs;
     // chirp-z path (n has a prime factor above 5): convolution of length blue_m

@@ -116,6 +116,18 @@ int gfftx_exec_oop(gfftx_plan plan, const void* src, void* dst, void* stream);
  * factory (src/gfftgen.h:222-231). */
 int gfftx_exec_batch(const gfftx_plan* plans, void* const* data, size_t count, void* stream);
 
+/* In-place transform that allocates nothing, for callers that cannot allocate during a call
+ * (code under graph capture, tight serving loops, harnesses that forbid allocation).  No
+ * counterpart in the reference.  As gfftx_exec_inplace on device pointers, except that the
+ * intermediate arrays of a multi-pass plan come from a static device array of the library
+ * (64 MiB per GPU, held until the process ends) instead of the plan's own scratch, and that
+ * no event, graph or staging buffer is created.  Limits: device pointers only; lengths of
+ * the factors 2, 3, 5 and 7 (chirp-z plans return GFFTX_ERR_UNSUPPORTED); a multi-pass
+ * plan's batch may hold at most 32 MiB (2^22 FP32 or 2^21 FP64 complex points; larger
+ * returns GFFTX_ERR_INVALID_ARG).  All such calls on one GPU share the static array: calls
+ * that overlap on one GPU (other streams, other threads) race, and the caller orders them. */
+int gfftx_exec_inplace_noalloc(gfftx_plan plan, void* data, void* stream);
+
 /* Wait until the work enqueued on `stream` (a hipStream_t; NULL: the null stream) has
  * finished.  No reference counterpart: GFFT's fft() is synchronous
  * (src/gfftpolicy.h:163-172, :203-212), and the C++ policy layer calls this after each
