@@ -341,7 +341,21 @@ def other_workloads(torch, api):
                                 "kernels": [(k, round(t, 1)) for k, t in ks],
                                 "kernel_events_sum_us": round(sum(t for _, t in ks), 1),
                                 "errors_vs_numpy_fp64": e3}
-    del z, o
+    # C2R, the inverse of C3: the n+1 bins back to 3*2^20 reals, the C2R pre-pass fused into
+    # the first pass; errors against numpy's irfft of the same bins (IRDFT inverts RDFT, as
+    # irfft inverts rfft) and against the C3 input (round trip)
+    zb = torch.empty(nr, dtype=torch.float32, device="cuda")
+    with api.Plan(nr // 2, api.FLOAT, api.IRDFT, api.OUT_OF_PLACE, flags=api.FLAG_R2C_BINS) as pd:
+        pd.fft(o, zb)
+        spec = o.cpu().numpy().astype(np.complex128)
+        ec = errors(zb.cpu().numpy().astype(np.complex128), np.fft.irfft(spec, n=nr))
+        ert = errors(zb.cpu().numpy().astype(np.complex128), xr.astype(np.float32).astype(np.complex128))
+        us = _call_time_us(torch, lambda: pd.fft(o, zb), iters=200)
+        kern = [k["kernel"] for k in json.loads(pd.describe())["passes"]]
+    log("C2R done")
+    res["c2r_3x2p20_f32"] = {"us": round(us, 1), "GBps_alg": round((8 * (nr // 2 + 1) + 4 * nr) / us / 1e3, 1),
+                             "kernels": kern, "errors_vs_numpy_fp64": ec, "round_trip_errors": ert}
+    del z, o, zb
     # C4: ragged batch of 4096 FP32 transforms, one call replayed as a captured hipGraph (the
     # workload's definition); the same calls issued directly are reported beside it
     lengths = [int(v) for v in inputs.ragged_lengths(4096, 4096)]

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgfftx.so")
 
-SOURCES = ["kernels.hip", "ragged.hip", "bluestein.hip", "planner.cpp", "plan.cpp", "host_pipeline.cpp",
+SOURCES = ["kernels.hip", "real_tiles.hip", "ragged.hip", "bluestein.hip", "planner.cpp", "plan.cpp", "host_pipeline.cpp",
            "tuner.cpp", "ragged_plan.cpp", "noalloc.cpp"]
 # per-source flags: the MFMA kernels keep their products in VGPRs (no accumulator copies)
 EXTRA = {"kernels.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}

@@ -1,58 +1,7 @@
 // kernels.hip -- instantiated FFT kernels for gfx950 and their host-visible registry.
-#include <hip/hip_ext.h>
-
-#include "registry.h"
+#include "tile_table.h"
 
 namespace gfftx {
-
-template <class C>
-static hipError_t launch_tile(const TileArgs& a, dim3 grid, hipStream_t s, hipEvent_t t0, hipEvent_t t1)
-{
-    if (t0) hipExtLaunchKernelGGL(tile_kernel<C>, grid, dim3(C::WG), 0, s, t0, t1, 0, a);
-    else hipLaunchKernelGGL(tile_kernel<C>, grid, dim3(C::WG), 0, s, a);
-    return hipGetLastError();
-}
-
-#define GFFTX_CFG(TYPE, L, NB, WG, LT, ST, ...) TileCfg<TYPE, L, NB, WG, LT, ST, __VA_ARGS__>
-#define GFFTX_TILE(PREC, TYPE, L, NB, WG, LT, ST, ...)                                         \
-    {PREC,                                                                                     \
-     L,                                                                                        \
-     NB,                                                                                       \
-     WG,                                                                                       \
-     LT,                                                                                       \
-     ST,                                                                                       \
-     GFFTX_CFG(TYPE, L, NB, WG, LT, ST, __VA_ARGS__)::NS,                                     \
-     GFFTX_CFG(TYPE, L, NB, WG, LT, ST, __VA_ARGS__)::R,                                      \
-     (int)(GFFTX_CFG(TYPE, L, NB, WG, LT, ST, __VA_ARGS__)::LDS_ELEMS * sizeof(V2<TYPE>)),    \
-     &launch_tile<GFFTX_CFG(TYPE, L, NB, WG, LT, ST, __VA_ARGS__)>,                           \
-     "tile_" #TYPE "_L" #L "_nb" #NB "_wg" #WG "_" #LT #ST,                                    \
-     false}
-
-// T->T pass tiles with the R2C split fused in (SplitCfg), one per tile shape of the last
-// passes of real forward transforms; NB must be even.
-#define GFFTX_SPLIT_TILE(PREC, TYPE, L, NB, WG, ...)                                                 \
-    {PREC,                                                                                             \
-     L,                                                                                                \
-     NB,                                                                                               \
-     WG,                                                                                               \
-     true,                                                                                             \
-     true,                                                                                             \
-     GFFTX_CFG(TYPE, L, NB, WG, true, true, __VA_ARGS__)::NS,                                        \
-     GFFTX_CFG(TYPE, L, NB, WG, true, true, __VA_ARGS__)::R,                                         \
-     (int)(SplitCfg<GFFTX_CFG(TYPE, L, NB, WG, true, true, __VA_ARGS__)>::LDS_ELEMS * sizeof(V2<TYPE>)), \
-     &launch_tile<SplitCfg<GFFTX_CFG(TYPE, L, NB, WG, true, true, __VA_ARGS__)>>,                    \
-     "tile_" #TYPE "_L" #L "_nb" #NB "_wg" #WG "_split",                                             \
-     true}
-
-#define GFFTX_SPLIT_SET(PREC, TYPE, NB4096, WG4096, NB1024, WG1024, NB1536, WG1536)                 \
-    GFFTX_SPLIT_TILE(PREC, TYPE, 16, 64, 64, 16),                                                     \
-    GFFTX_SPLIT_TILE(PREC, TYPE, 64, 64, 256, 16, 4),                                                 \
-    GFFTX_SPLIT_TILE(PREC, TYPE, 128, 32, 256, 16, 8),                                                \
-    GFFTX_SPLIT_TILE(PREC, TYPE, 256, 16, 256, 16, 16),                                               \
-    GFFTX_SPLIT_TILE(PREC, TYPE, 320, 12, 240, 16, 4, 5),                                             \
-    GFFTX_SPLIT_TILE(PREC, TYPE, 1024, NB1024, WG1024, 16, 16, 4),                                    \
-    GFFTX_SPLIT_TILE(PREC, TYPE, 1536, NB1536, WG1536, 16, 16, 2, 3),                                 \
-    GFFTX_SPLIT_TILE(PREC, TYPE, 4096, NB4096, WG4096, 16, 16, 16)
 
 // Batched single-pass (E->E) tiles.  TPT = L/16 threads per sequence (one radix-16
 // butterfly each), NB sequences per workgroup with NB*TPT close to 256.  Radix-2/4/8/3/5
@@ -186,20 +135,13 @@ TileEntry kTiles[] = {
     GFFTX_TILE(PREC_F32, float, 1536, 2, 192, true, true, 16, 16, 2, 3),
     GFFTX_TILE(PREC_F32, float, 1024, 2, 128, true, false, 16, 16, 4),
     GFFTX_TILE(PREC_F32, float, 1024, 2, 128, true, true, 16, 16, 4),
-    GFFTX_SPLIT_TILE(PREC_F32, float, 1536, 2, 192, 16, 16, 2, 3),
-    GFFTX_SPLIT_TILE(PREC_F32, float, 1536, 4, 384, 16, 16, 2, 3),
-    GFFTX_SPLIT_TILE(PREC_F32, float, 1024, 4, 256, 16, 16, 4),
-    GFFTX_SPLIT_TILE(PREC_F32, float, 1024, 2, 128, 16, 16, 4),
     // eight points per thread (radix-8 stages): twice the waves of the radix-16 tiles
     GFFTX_TILE(PREC_F32, float, 1536, 4, 768, true, false, 8, 8, 8, 3),
     GFFTX_TILE(PREC_F32, float, 1536, 4, 768, true, true, 8, 8, 8, 3),
-    GFFTX_SPLIT_TILE(PREC_F32, float, 1536, 4, 768, 8, 8, 8, 3),
     GFFTX_TILE(PREC_F32, float, 1024, 8, 1024, true, false, 8, 8, 8, 2),
     GFFTX_TILE(PREC_F32, float, 1024, 8, 1024, true, true, 8, 8, 8, 2),
-    GFFTX_SPLIT_TILE(PREC_F32, float, 1024, 8, 1024, 8, 8, 8, 2),
     GFFTX_TILE(PREC_F32, float, 1024, 4, 512, true, false, 8, 8, 8, 2),
     GFFTX_TILE(PREC_F32, float, 1024, 4, 512, true, true, 8, 8, 8, 2),
-    GFFTX_SPLIT_TILE(PREC_F32, float, 1024, 4, 512, 8, 8, 8, 2),
     GFFTX_TILE(PREC_F32, float, 16, 64, 64, true, true, 16),
     GFFTX_TILE(PREC_F32, float, 64, 64, 256, true, true, 16, 4),
     GFFTX_TILE(PREC_F32, float, 16, 64, 64, true, false, 16),
@@ -208,7 +150,6 @@ TileEntry kTiles[] = {
     GFFTX_PASS_SMALL(PREC_F32, float),
     GFFTX_BATCH_7(PREC_F32, float),
     GFFTX_PASS_7(PREC_F32, float),
-    GFFTX_SPLIT_SET(PREC_F32, float, 4, 1024, 8, 512, 8, 768),
     GFFTX_BATCH_POW2(PREC_F64, double),
     GFFTX_BATCH_MIXED(PREC_F64, double),
     GFFTX_PASS_SET(PREC_F64, double, 2, 512, 4, 256, 4, 384),
@@ -229,7 +170,6 @@ TileEntry kTiles[] = {
     GFFTX_PASS_SMALL(PREC_F64, double),
     GFFTX_BATCH_7(PREC_F64, double),
     GFFTX_PASS_7(PREC_F64, double),
-    GFFTX_SPLIT_SET(PREC_F64, double, 2, 512, 4, 256, 4, 384),
 };
 const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
@@ -241,9 +181,13 @@ static hipError_t launch_mfma(const MfmaArgs& a, dim3 grid, hipStream_t s, hipEv
     return hipGetLastError();
 }
 
+// Per-stage scaling first (the default); the per-column variants remain for comparison
+// (GFFTX_MFMA_SCALE=column).
 MfmaEntry kMfma[] = {
-    {256, 16, &launch_mfma<MfmaCfg<256, 16>>, "mfma_split_f32_L256_nb16"},
-    {4096, 1, &launch_mfma<MfmaCfg<4096, 1>>, "mfma_split_f32_L4096_nb1"},
+    {256, 16, MFMA_SCALE_STAGE, &launch_mfma<MfmaCfg<256, 16>>, "mfma_split_f32_L256_nb16"},
+    {4096, 1, MFMA_SCALE_STAGE, &launch_mfma<MfmaCfg<4096, 1>>, "mfma_split_f32_L4096_nb1"},
+    {256, 16, MFMA_SCALE_COLUMN, &launch_mfma<MfmaCfg<256, 16, MFMA_SCALE_COLUMN>>, "mfma_split_f32_L256_nb16_col"},
+    {4096, 1, MFMA_SCALE_COLUMN, &launch_mfma<MfmaCfg<4096, 1, MFMA_SCALE_COLUMN>>, "mfma_split_f32_L4096_nb1_col"},
 };
 const int kNumMfma = sizeof(kMfma) / sizeof(kMfma[0]);
 

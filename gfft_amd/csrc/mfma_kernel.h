@@ -9,8 +9,16 @@
 // Each 16-column block is two v_mfma_f32_16x16x32_f16 products (A0 and A1 against one
 // B operand).  FP32 accuracy is recovered with a hi/lo split of both operands,
 //     A B ~= Ah Bh + Ah Bl + Al Bh        (FP32 accumulation, smallest terms first),
-// and every column is scaled by a power of two before the split so that its largest entry
-// sits just below 2^14: the FP16 parts stay in range and the scaling is exact.
+// with the data scaled by powers of two (exact) so that the FP16 parts stay in range:
+//   - per stage (MFMA_SCALE_STAGE, the default): the first stage scales the transform by
+//     s0 = 2^(14-e), e the exponent of its largest component, so every input component is
+//     below 2^14; a radix-16 DFT grows a component at most 16 sqrt(2) times, so each later
+//     stage takes its input times 2^-4 (below 2^14.5 and 2^15 for the second and third
+//     stages), folded into its twiddle table; the last store multiplies by
+//     2^(4 (stages-1)) / s0.  Per point and stage this costs the split alone (the scale
+//     rides on the twiddle product);
+//   - per column (MFMA_SCALE_COLUMN): every column of every stage is scaled so that its
+//     largest entry sits just below 2^14 and unscaled after the product.
 //
 // Lane layout of the 16x16x32 f16 product on gfx950 (lane l, i = 0..7 / 0..3):
 //     A: A[l%16][8(l/16)+i]   B: B[8(l/16)+i][l%16]   C: C[4(l/16)+i][l%16]
@@ -30,8 +38,11 @@ typedef float float4_t __attribute__((ext_vector_type(4)));
 // part = 0 (hi) / 1 (lo).
 constexpr int MFMA_A_HALFS = 2 * 2 * 64 * 8;
 
-template <int L_, int NB_> struct MfmaCfg {
+enum { MFMA_SCALE_STAGE = 0, MFMA_SCALE_COLUMN = 1 };
+
+template <int L_, int NB_, int SCALE_ = MFMA_SCALE_STAGE> struct MfmaCfg {
     static constexpr int L = L_, NB = NB_, WG = 256;
+    static constexpr int SCALE = SCALE_;
     static constexpr int COLS = L / 16;                 // columns per sequence and stage
     static constexpr int BLOCKS = NB * COLS / 16;       // 16-column blocks per stage (all sequences)
     static constexpr int BPW = BLOCKS / 4;              // blocks per wave
@@ -60,7 +71,8 @@ struct MfmaArgs {
     const float2* in;
     float2* out;
     long long ntrans;
-    const float2* tw;       // stage twiddles, rows of 16 entries (same layout as the tile kernel)
+    const float2* tw;       // stage twiddles, rows of 16 entries (same layout as the tile kernel);
+                            // times 2^-4 for MFMA_SCALE_STAGE kernels
     const _Float16* amat;   // MFMA_A_HALFS halves
     int conj_in;
     int conj_out;
@@ -81,8 +93,31 @@ __device__ __forceinline__ float col_max4(float m)
     return __uint_as_float(u);
 }
 
+// Largest value over the wave (m >= 0: compared as unsigned bit patterns).
+__device__ __forceinline__ float wave_max(float m)
+{
+    unsigned u = __float_as_uint(m);
+    const auto a = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+    u = max(a[0], a[1]);
+    const auto b = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+    u = max(b[0], b[1]);
+#pragma unroll
+    for (int d = 8; d >= 1; d >>= 1) u = max(u, (unsigned)__shfl_xor((int)u, d, 64));
+    return __uint_as_float(u);
+}
+
+// s0 = 2^(14 - e) for the largest component m = f 2^e (f in [0.5, 1)) of a transform, the
+// exponent clamped so that s0 and its inverse are normal FP32 numbers; returns log2 s0.
+__device__ __forceinline__ int stage_shift(float m)
+{
+    int ex = 0;
+    (void)frexpf(m, &ex);
+    return min(max(14 - ex, -126), 126);
+}
+
 template <class C, int S>
-__device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const half8_t (&A)[2][2], long long t0)
+__device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const half8_t (&A)[2][2], long long t0,
+                                           int (&sh)[C::BPW])
 {
     constexpr int NSS = C::ns(S);
     constexpr bool FIRST = (S == 0), LAST = (S == C::NS - 1);
@@ -90,6 +125,62 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
     const int c = lane & 15, g = lane >> 4;
     float2 y[C::BPW][4];
     const unsigned cmask_in = a.conj_in ? 0x80000000u : 0u;
+    // first stage of per-stage scaling: all inputs are loaded first, then the largest
+    // component of each transform sets its scale
+    float2 x0[FIRST ? C::BPW : 1][4];
+    if constexpr (FIRST) {
+#pragma unroll
+        for (int b = 0; b < C::BPW; ++b) {
+            const int blk = wave * C::BPW + b;
+            const int t = blk / (C::COLS / 16);
+            const int j = (blk % (C::COLS / 16)) * 16 + c;
+            const long long tg = t0 + t;
+            // loads past the last sequence are clamped to it; their results are not stored.
+            // The input is read once, so it is loaded non-temporally (whole sequences).
+            const float2* src = a.in + (tg < a.ntrans ? tg : a.ntrans - 1) * C::L;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                x0[b][i] = flip_im(nt_load(&src[(unsigned)(j + (4 * g + i) * C::COLS)]), cmask_in);
+        }
+        if constexpr (C::SCALE == MFMA_SCALE_STAGE) {
+            // blocks per sequence: COLS/16; a wave holds BPW blocks, so a sequence spans
+            // (COLS/16)/BPW waves (one: the wave's blocks are whole sequences)
+            constexpr int BPS = C::COLS / 16;
+            if constexpr (BPS >= C::BPW) {
+                static_assert(BPS % C::BPW == 0, "whole waves per sequence");
+                float m = 0.f;
+#pragma unroll
+                for (int b = 0; b < C::BPW; ++b)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) m = fmaxf(fmaxf(m, fabsf(x0[b][i].x)), fabsf(x0[b][i].y));
+                m = wave_max(m);
+                constexpr int WPS = BPS / C::BPW;  // waves per sequence
+                if constexpr (WPS > 1) {
+                    // the waves of a sequence meet in the LDS, before the stage writes it
+                    float* red = reinterpret_cast<float*>(lds);
+                    if (lane == 0) red[wave] = m;
+                    __syncthreads();
+                    const int w0 = (wave / WPS) * WPS;
+#pragma unroll
+                    for (int w = 0; w < WPS; ++w) m = fmaxf(m, red[w0 + w]);
+                    __syncthreads();
+                }
+                const int e = stage_shift(m);
+#pragma unroll
+                for (int b = 0; b < C::BPW; ++b) sh[b] = e;
+            } else {
+                // several sequences per wave, a block each: the largest component of each block
+                static_assert(BPS == 1, "a block is a whole sequence");
+#pragma unroll
+                for (int b = 0; b < C::BPW; ++b) {
+                    float m = 0.f;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) m = fmaxf(fmaxf(m, fabsf(x0[b][i].x)), fabsf(x0[b][i].y));
+                    sh[b] = stage_shift(wave_max(m));
+                }
+            }
+        }
+    }
 
 #pragma unroll
     for (int b = 0; b < C::BPW; ++b) {
@@ -99,38 +190,57 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
         const long long tg = t0 + t;
         float2 v[4];
         if constexpr (FIRST) {
-            // loads past the last sequence are clamped to it; their results are not stored.
-            // The input is read once, so it is loaded non-temporally (whole sequences).
-            const float2* src = a.in + (tg < a.ntrans ? tg : a.ntrans - 1) * C::L;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) v[i] = flip_im(nt_load(&src[(unsigned)(j + (4 * g + i) * C::COLS)]), cmask_in);
+            for (int i = 0; i < 4; ++i) v[i] = x0[b][i];
         } else {
+            // lidx(t, j + m COLS) with m = 4g + i: COLS is a multiple of 16 and j = 16 jb + c,
+            // so the padding term (e >> 4) is jb + m COLS/16 and the rows are RS apart
+            constexpr int RS = C::COLS + C::COLS / 16;
+            const float2* rd = lds + t * C::TS + j + (j >> 4) + 4 * g * RS;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) v[i] = lds[C::lidx(t, j + (4 * g + i) * C::COLS)];
+            for (int i = 0; i < 4; ++i) v[i] = rd[i * RS];
         }
         if constexpr (!FIRST) {
             const int k = j % NSS;
             const float2* row = a.tw + C::twoff(S) + k * 16 + 4 * g;
+            if constexpr (C::SCALE == MFMA_SCALE_STAGE) {
+                // the row carries the stage's 2^-4, so W^0 is not 1 here
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-                if (g != 0 || i != 0) v[i] = cmul(v[i], row[i]);
+                for (int i = 0; i < 4; ++i) v[i] = cmul(v[i], row[i]);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (g != 0 || i != 0) v[i] = cmul(v[i], row[i]);
+            }
         }
-        // per-column power-of-two scale: largest entry just below 2^14 (the exponent is
-        // clamped so that the scale and its inverse are normal FP32 numbers)
-        float m = 0.f;
+        float sc = 1.0f, inv = 1.0f;
+        if constexpr (C::SCALE == MFMA_SCALE_COLUMN) {
+            // per-column power-of-two scale: largest entry just below 2^14 (the exponent is
+            // clamped so that the scale and its inverse are normal FP32 numbers)
+            float m = 0.f;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) m = fmaxf(fmaxf(m, fabsf(v[i].x)), fabsf(v[i].y));  // one v_max3 per point
-        m = col_max4(m);
-        int ex = 0;
-        (void)frexpf(m, &ex);
-        const int sh = min(max(14 - ex, -126), 126);
-        const float sc = ldexpf(1.0f, sh), inv = ldexpf(1.0f, -sh);
+            for (int i = 0; i < 4; ++i) m = fmaxf(fmaxf(m, fabsf(v[i].x)), fabsf(v[i].y));  // one v_max3 per point
+            m = col_max4(m);
+            int ex = 0;
+            (void)frexpf(m, &ex);
+            const int csh = min(max(14 - ex, -126), 126);
+            sc = ldexpf(1.0f, csh);
+            inv = ldexpf(1.0f, -csh);
+        } else if constexpr (FIRST) {
+            sc = ldexpf(1.0f, sh[b]);
+        } else {
+            // s = 1, opaque to the compiler: x - hi then stays one mixed-precision FMA per
+            // component instead of a conversion of hi and a subtraction
+            asm volatile("" : "+v"(sc));
+        }
         half8_t bh, bl;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             // hi = FP16(x s); lo = FP16(x s - hi): x s - hi is formed by one mixed-precision
-            // FMA from x, s and the FP16 hi (the product by a power of two is exact)
-            const float2_t xs = {v[i].x * sc, v[i].y * sc};
+            // FMA from x, s and the FP16 hi (the product by a power of two is exact; s = 1
+            // after the first stage of per-stage scaling)
+            const float2_t xs = (C::SCALE == MFMA_SCALE_STAGE && !FIRST) ? float2_t{v[i].x, v[i].y}
+                                                                         : float2_t{v[i].x * sc, v[i].y * sc};
             const half2_t h = __builtin_convertvector(xs, half2_t);
             bh[2 * i] = h.x;
             bh[2 * i + 1] = h.y;
@@ -146,8 +256,13 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
             z = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[mt][0], bh, z, 0, 0, 0);
             acc[mt] = z;
         }
+        if constexpr (C::SCALE == MFMA_SCALE_COLUMN) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) y[b][i] = float2{acc[0][i] * inv, acc[1][i] * inv};
+            for (int i = 0; i < 4; ++i) y[b][i] = float2{acc[0][i] * inv, acc[1][i] * inv};
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) y[b][i] = float2{acc[0][i], acc[1][i]};
+        }
     }
 
     if constexpr (!FIRST) __syncthreads();
@@ -164,7 +279,8 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
             const long long tg = t0 + t;
             if (tg < a.ntrans) {
                 float2* dst = a.out + tg * C::L;
-                const float sc = a.scale;
+                // per-stage scaling: undo s0 and the 2^-4 of each later stage
+                const float sc = C::SCALE == MFMA_SCALE_STAGE ? ldexpf(a.scale, 4 * (C::NS - 1) - sh[b]) : a.scale;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     float2 u = y[b][i];
@@ -174,19 +290,26 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
                 }
             }
         } else {
+            // lidx(t, base + m NSS), m = 4g + i: for NSS = 1, base = 16 j and m < 16; for NSS
+            // a multiple of 16 the padding term grows by NSS/16 per m.  Either way the
+            // elements are WS apart from t TS + base + (base >> 4)
+            constexpr int WS = NSS == 1 ? 1 : NSS + NSS / 16;
+            static_assert(NSS == 1 || NSS % 16 == 0, "radix-16 stages");
+            float2* wr = lds + t * C::TS + base + (base >> 4) + 4 * g * WS;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) lds[C::lidx(t, base + (4 * g + i) * NSS)] = y[b][i];
+            for (int i = 0; i < 4; ++i) wr[i * WS] = y[b][i];
         }
     }
     if constexpr (!LAST) __syncthreads();
 }
 
 template <class C, int S>
-__device__ __forceinline__ void mfma_loop(const MfmaArgs& a, float2* lds, const half8_t (&A)[2][2], long long t0)
+__device__ __forceinline__ void mfma_loop(const MfmaArgs& a, float2* lds, const half8_t (&A)[2][2], long long t0,
+                                          int (&sh)[C::BPW])
 {
     if constexpr (S < C::NS) {
-        mfma_stage<C, S>(a, lds, A, t0);
-        mfma_loop<C, S + 1>(a, lds, A, t0);
+        mfma_stage<C, S>(a, lds, A, t0, sh);
+        mfma_loop<C, S + 1>(a, lds, A, t0, sh);
     }
 }
 
@@ -201,7 +324,8 @@ template <class C> __global__ __launch_bounds__(256) void mfma_fft_kernel(MfmaAr
         for (int p = 0; p < 2; ++p)
 #pragma unroll
             for (int i = 0; i < 8; ++i) A[mt][p][i] = a.amat[((mt * 2 + p) * 64 + lane) * 8 + i];
-    mfma_loop<C, 0>(a, lds, A, (long long)blockIdx.x * C::NB);
+    int sh[C::BPW] = {};  // log2 of each block's first-stage scale (per-stage scaling)
+    mfma_loop<C, 0>(a, lds, A, (long long)blockIdx.x * C::NB, sh);
 }
 
 }  // namespace gfftx

@@ -63,8 +63,10 @@ int build_c2c_passes(gfftx_plan_s* p)
               ",\"precision\":\"" + (p->prec == PREC_F64 ? "f64" : "f32") + "\",\"passes\":[";
     if (p->flags & GFFTX_FLAG_SPLIT_MFMA) {
         const MfmaEntry* me = nullptr;
+        const char* sm = getenv("GFFTX_MFMA_SCALE");
+        const int want = (sm && !strcmp(sm, "column")) ? MFMA_SCALE_COLUMN : MFMA_SCALE_STAGE;
         for (int i = 0; i < kNumMfma; ++i)
-            if ((unsigned long long)kMfma[i].L == n) me = &kMfma[i];
+            if ((unsigned long long)kMfma[i].L == n && kMfma[i].scale == want) me = &kMfma[i];
         if (p->prec != PREC_F32 || p->real || !me)
             return fail(GFFTX_ERR_UNSUPPORTED, "split-precision MFMA path: FP32 C2C of length 256 or 4096 only (n=%llu)", n);
         Pass ps;
@@ -72,7 +74,10 @@ int build_c2c_passes(gfftx_plan_s* p)
         ps.items_in_ntrans = true;
         ps.ntrans = (long long)p->batch;
         const int rad[4] = {16, 16, 16, 16};
-        int rc = upload_stage_table_r(p->allocs, PREC_F32, me->L == 256 ? 2 : 3, rad, &ps.stage_tw);
+        // per-stage scaling: each stage after the first takes its input times 2^-4 through
+        // the twiddles (exact: a power of two)
+        int rc = upload_stage_table_r(p->allocs, PREC_F32, me->L == 256 ? 2 : 3, rad, &ps.stage_tw,
+                                      me->scale == MFMA_SCALE_STAGE ? 0.0625 : 1.0);
         if (rc) return rc;
         rc = upload_mfma_matrix(p->allocs, &ps.amat);
         if (rc) return rc;
@@ -105,8 +110,19 @@ int build_c2c_passes(gfftx_plan_s* p)
         ps.batch = 1;
         int rc = upload_stage_table(p, k, &ps.stage_tw);
         if (rc) return rc;
+        // real transform: the split runs inside the tile (after the last stage forward, before
+        // the first stage inverse) when the tile has such a variant
+        if (p->real && env_int("GFFTX_FUSED_SPLIT", 1)) {
+            if (const TileEntry* kr = find_real_tile(k, p->inverse ? REAL_PRE : REAL_POST)) {
+                ps.k = kr;
+                ps.real = kr->real;
+                rc = upload_root_tables(p->allocs, (long long)(2 * n), &ps.sp_lo, &ps.sp_hi, &ps.sp_shift);
+                if (rc) return rc;
+                p->split_fused = true;
+            }
+        }
         p->passes.push_back(ps);
-        snprintf(buf, sizeof(buf), "{\"kernel\":\"%s\",\"L\":%d,\"ntrans\":%lld}", k->name, k->L, ps.ntrans);
+        snprintf(buf, sizeof(buf), "{\"kernel\":\"%s\",\"L\":%d,\"ntrans\":%lld}", ps.k->name, k->L, ps.ntrans);
         p->desc += buf;
     } else {
         std::vector<int> lens;
@@ -137,14 +153,15 @@ int build_c2c_passes(gfftx_plan_s* p)
                 if (rc) return rc;
             }
             p->passes.push_back(ps);
-            // real forward transform: the last pass also computes the R2C split when its
-            // tile has a fused variant
-            if (i + 1 == lens.size() && p->real && !p->inverse && env_int("GFFTX_FUSED_SPLIT", 1)) {
-                if (const TileEntry* ks = find_split_tile(k)) {
-                    Pass& last = p->passes.back();
-                    last.k = ks;
-                    last.split = true;
-                    rc = upload_root_tables(p->allocs, (long long)(2 * n), &last.sp_lo, &last.sp_hi, &last.sp_shift);
+            // real transform: the last pass of a forward transform computes the R2C split, the
+            // first pass of an inverse one the C2R pre-pass, when the tile has a fused variant
+            const bool fuse_here = p->inverse ? i == 0 : i + 1 == lens.size();
+            if (fuse_here && p->real && env_int("GFFTX_FUSED_SPLIT", 1)) {
+                if (const TileEntry* ks = find_real_tile(k, p->inverse ? REAL_PRE : REAL_POST)) {
+                    Pass& pr = p->passes.back();
+                    pr.k = ks;
+                    pr.real = ks->real;
+                    rc = upload_root_tables(p->allocs, (long long)(2 * n), &pr.sp_lo, &pr.sp_hi, &pr.sp_shift);
                     if (rc) return rc;
                     p->split_fused = true;
                 }
@@ -301,15 +318,18 @@ int run_passes(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, Range 
         a.xcd = ps.xcd;
         const long long nb = ps.k->nb;
         unsigned ntiles = (unsigned)((a.ntrans + nb - 1) / nb);
-        if (ps.split) {
-            // tile 0: sequences 0 and S/2; then nb/2 pairs (p, S - p), 1 <= p <= (S-1)/2
+        if (ps.real) {
             a.sp_lo = ps.sp_lo;
             a.sp_hi = ps.sp_hi;
             a.sp_shift = ps.sp_shift;
             a.split_bins = (p->flags & GFFTX_FLAG_R2C_BINS) ? 1 : 0;
             a.nfull = (long long)p->n;
-            const long long pairs = (a.ntrans - 1) / 2, h = nb / 2;
-            ntiles = (unsigned)(1 + (pairs + h - 1) / h);
+            if (ps.k->loadT) {
+                // paired T-mode tiles: tile 0 takes sequences 0 and S/2, then nb/2 pairs
+                // (p, S - p), 1 <= p <= (S-1)/2
+                const long long pairs = (a.ntrans - 1) / 2, h = nb / 2;
+                ntiles = (unsigned)(1 + (pairs + h - 1) / h);
+            }
         }
         dim3 grid(ntiles, ps.items_in_ntrans ? 1u : (unsigned)r.nb);
         std::pair<hipEvent_t, hipEvent_t> ev;
@@ -402,16 +422,17 @@ int run_split(gfftx_plan_s* p, void* data, hipStream_t s, Range r)
 
 // Full transform on device pointers.  Real forward: complex FFT then split on the output.
 // Real inverse: split on a copy of the input (the input of an out-of-place call is left
-// unchanged), then the inverse complex FFT.
+// unchanged), then the inverse complex FFT.  A plan whose split is fused into one of its
+// passes runs the complex passes only.
 // `r` selects batch items (all by default); in and out are the base pointers of the batch.
 int exec_device(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, Range r)
 {
     if (!r.nb) r.nb = p->batch;
-    if (!p->real) return run_c2c(p, in, out, s, r);
+    if (!p->real || p->split_fused) return run_c2c(p, in, out, s, r);
     if (!p->inverse) {
         int rc = run_c2c(p, in, out, s, r);
         if (rc) return rc;
-        return p->split_fused ? GFFTX_SUCCESS : run_split(p, out, s, r);
+        return run_split(p, out, s, r);
     }
     void* work = out;
     if (in != out) {
@@ -476,20 +497,15 @@ int exec_dispatch(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
 }
 
 // One call of the plan: calls from several threads run one at a time, and a call on another
-// stream than the previous call's waits for that call's work on the device (the scratch
-// arrays are shared).
+// stream than the previous call's waits until that call's work is done when the calls share
+// device arrays (scratch, chirp-z work, graphs).
 int exec_any(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
 {
     std::lock_guard<std::mutex> lk(p->mu);
     DeviceGuard g(p->device);
     if (!g.ok) return fail(GFFTX_ERR_DEVICE, "cannot select device %d", p->device);
-    HIP_TRY(p->order.before(s));
-    const int rc = exec_dispatch(p, in, out, s);
-    // recorded also after a failed call, which may have enqueued part of its work
-    const hipError_t oe = p->order.after(s);
-    if (rc) return rc;
-    if (oe != hipSuccess) return fail(GFFTX_ERR_DEVICE, "stream order event: %s", hipGetErrorString(oe));
-    return GFFTX_SUCCESS;
+    if (p->shared_state) HIP_TRY(p->order.before(s));
+    return exec_dispatch(p, in, out, s);
 }
 
 // Moves the scratch buffers of `src` into `dst` where both have one (same length, batch and
@@ -613,8 +629,10 @@ int create_plan_impl(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int
     p->forced_lens = forced;
     p->tune_note = note;
     int rc = native_length(p->prec, n) ? build_c2c_passes(p) : build_bluestein(p);
-    if (rc == GFFTX_SUCCESS && real) rc = upload_split_table(p);
-    if (rc == GFFTX_SUCCESS && real && p->inverse)  // split pass of an out-of-place IRDFT works on a copy
+    // an unfused split runs as a kernel of its own with a table; that of an out-of-place
+    // IRDFT works on a copy of the input
+    if (rc == GFFTX_SUCCESS && real && !p->split_fused) rc = upload_split_table(p);
+    if (rc == GFFTX_SUCCESS && real && p->inverse && !p->split_fused)
         rc = dev_alloc(p, &p->real_work, (size_t)p->dist * batch * elem_bytes(p->prec));
     if (rc == GFFTX_SUCCESS && p->passes.size() > 1) {
         const size_t bytes = (size_t)p->dist * batch * elem_bytes(p->prec);
@@ -625,6 +643,8 @@ int create_plan_impl(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int
         gfftx_plan_destroy(p);  // frees inner plans too; leaves the error message alone
         return rc;
     }
+    // device arrays that every call writes: calls from different streams must not overlap
+    p->shared_state = p->scratch[0] || p->blue_fwd || p->real_work || (flags & GFFTX_FLAG_GRAPH);
     *out = p;
     return GFFTX_SUCCESS;
 }
@@ -722,7 +742,6 @@ int gfftx_plan_destroy(gfftx_plan plan)
     if (!plan) return GFFTX_SUCCESS;
     DeviceGuard g(plan->device);
     plan->graphs.clear();
-    plan->order.clear();
     for (int k = 0; k < 2; ++k) {
         if (plan->pin_in[k]) (void)hipHostFree(plan->pin_in[k]);
         if (plan->pin_out[k]) (void)hipHostFree(plan->pin_out[k]);

@@ -62,7 +62,7 @@ struct Pass {
     long long ptw_m = 0;
     int xcd = 0;
     bool items_in_ntrans = false;  // single-pass and MFMA: one sequence per batch item
-    bool split = false;            // SplitCfg tile: the R2C split fused into this last pass
+    int real = REAL_NONE;          // RealCfg tile: the real-transform split fused in
     double2* sp_lo = nullptr;      // its W_{2N} table
     double2* sp_hi = nullptr;
     int sp_shift = 0;
@@ -76,7 +76,7 @@ struct Range {
 inline size_t elem_bytes(int prec) { return prec == PREC_F64 ? 16 : 8; }
 
 const TileEntry* find_tile(int prec, int L, bool loadT, bool storeT, int want_nb = 0, int want_wg = 0);
-const TileEntry* find_split_tile(const TileEntry* k);
+const TileEntry* find_real_tile(const TileEntry* k, int kind);
 int env_int(const char* name, int dflt);
 int device_cus(int device);
 double pass_bandwidth(int prec, int L);
@@ -90,7 +90,8 @@ bool choose_passes(int prec, unsigned long long n, std::vector<int>& out, unsign
 unsigned long long bluestein_max_m(int prec);
 unsigned long long bluestein_length(int prec, unsigned long long n, unsigned long long batch);
 int dev_alloc_list(std::vector<void*>& owned, void** out, size_t bytes);
-int upload_stage_table_r(std::vector<void*>& owned, int prec, int nrad, const int* radices, void** out);
+int upload_stage_table_r(std::vector<void*>& owned, int prec, int nrad, const int* radices, void** out,
+                         double scale = 1.0);
 int upload_root_tables(std::vector<void*>& owned, long long M, double2** lo_out, double2** hi_out, int* shift);
 int upload_mfma_matrix(std::vector<void*>& owned, void** out);
 
@@ -104,7 +105,6 @@ struct GraphCache {
         void* out = nullptr;
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
-        hipEvent_t done = nullptr;  // recorded after each launch of `exec`
         unsigned long long used = 0;
     };
     Entry e[kSlots];
@@ -120,7 +120,9 @@ struct GraphCache {
             }
         return nullptr;
     }
-    // frees the least recently used entry (or picks an empty one) and returns it
+    // frees the least recently used entry (or picks an empty one) and returns it; an entry
+    // that was launched is destroyed only after the device has finished its work (a rare
+    // path: more than kSlots buffer pairs in rotation)
     Entry* take()
     {
         Entry* v = &e[0];
@@ -131,34 +133,24 @@ struct GraphCache {
             }
             if (x.used < v->used) v = &x;
         }
+        if (v->exec) (void)hipDeviceSynchronize();
         release(*v);
         return v;
     }
-    // waits for the entry's last launch before its executable graph is destroyed
     static void release(Entry& x)
     {
-        if (x.done) {
-            (void)hipEventSynchronize(x.done);
-            (void)hipEventDestroy(x.done);
-        }
         if (x.exec) (void)hipGraphExecDestroy(x.exec);
         if (x.graph) (void)hipGraphDestroy(x.graph);
         (void)hipGetLastError();
         x = Entry{};
     }
-    // launches the entry's graph on s and records its completion event
-    static hipError_t launch(Entry& x, hipStream_t s)
-    {
-        if (!x.done) {
-            const hipError_t e = hipEventCreateWithFlags(&x.done, hipEventDisableTiming);
-            if (e != hipSuccess) return e;
-        }
-        const hipError_t e = hipGraphLaunch(x.exec, s);
-        if (e != hipSuccess) return e;
-        return hipEventRecord(x.done, s);
-    }
+    static hipError_t launch(Entry& x, hipStream_t s) { return hipGraphLaunch(x.exec, s); }
+    // the plan is being destroyed: its launched graphs finish before they are released
     void clear()
     {
+        bool launched = false;
+        for (auto& x : e) launched = launched || x.exec;
+        if (launched) (void)hipDeviceSynchronize();
         for (auto& x : e) release(x);
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
         cap_stream = nullptr;
@@ -167,44 +159,28 @@ struct GraphCache {
 
 // Orders the calls of one plan that arrive on different streams.  A plan's scratch arrays,
 // staging buffers and graphs serve every call, so a call issued on another stream than the
-// previous one first waits (on the device, hipStreamWaitEvent) for the previous call's work.
-// Calls on one stream are ordered by the stream and pay one event record each.  A call on a
-// stream that the caller is capturing into a graph neither waits nor records: the caller
-// orders the replays of its own graph.
+// previous call's first waits until the device has finished its earlier work, that call's
+// included (hipDeviceSynchronize: the previous stream may no longer exist, so nothing is
+// recorded on it).  Calls on one stream are ordered by the stream and cost nothing here: an
+// event recorded after every call measured 3-6 us per call on the device timeline.  A call
+// on a stream that the caller is capturing into a graph is not ordered this way (no host
+// wait is allowed there); the caller orders the replays of its own graphs.
 struct StreamOrder {
-    hipEvent_t ev = nullptr;
     hipStream_t last = nullptr;
-    bool valid = false;  // ev holds the previous call's completion
-    bool capturing = false;
-    // before a call on s; sets `capturing` for after()
+    bool used = false;
     hipError_t before(hipStream_t s)
     {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         hipError_t e = hipStreamIsCapturing(s, &cs);
         if (e != hipSuccess) return e;
-        capturing = cs != hipStreamCaptureStatusNone;
-        if (capturing || !valid || s == last) return hipSuccess;
-        return hipStreamWaitEvent(s, ev, 0);
-    }
-    // after the call's work has been enqueued on s
-    hipError_t after(hipStream_t s)
-    {
-        if (capturing) return hipSuccess;
-        if (!ev) {
-            const hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (cs != hipStreamCaptureStatusNone) return hipSuccess;
+        if (used && s != last) {
+            e = hipDeviceSynchronize();
             if (e != hipSuccess) return e;
         }
-        const hipError_t e = hipEventRecord(ev, s);
-        if (e != hipSuccess) return e;
         last = s;
-        valid = true;
+        used = true;
         return hipSuccess;
-    }
-    void clear()
-    {
-        if (ev) (void)hipEventDestroy(ev);
-        ev = nullptr;
-        valid = false;
     }
 };
 
@@ -250,7 +226,8 @@ struct gfftx_plan_s {
     std::vector<void*> allocs;     // device allocations owned by the plan
     void* scratch[2] = {nullptr, nullptr};
     void* split_tw = nullptr;
-    bool split_fused = false;      // real forward: the split runs inside the last pass
+    bool split_fused = false;      // real transform: the split runs inside a pass (last
+                                   // pass forward, first pass inverse)
     void* real_work = nullptr;     // IRDFT out-of-place: split runs on a copy of src
     DevBuf stage_in, stage_out;    // device arrays a host-pointer call runs on
     // host-pointer pipeline: two pinned buffers each way, a stream for each copy direction
@@ -285,7 +262,8 @@ struct gfftx_plan_s {
     // calls from several threads run one at a time: the graph cache, the profiling slots and
     // the host staging are the plan's own
     std::mutex mu;
-    StreamOrder order;  // calls from several streams wait for each other on the device
+    StreamOrder order;          // calls from several streams wait for each other
+    bool shared_state = false;  // calls write plan-owned device arrays (scratch, graphs)
 };
 
 // ---------------------------------------------------------------------------------------
@@ -328,7 +306,7 @@ struct gfftx_ragged_s {
     // profiling (GFFTX_FLAG_PROFILE): events around the launch
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending, pool;
     std::mutex mu;  // calls from several threads run one at a time
-    StreamOrder order;  // calls from several streams wait for each other on the device
+    StreamOrder order;  // calls from several streams wait for each other
 };
 
 namespace gfftx {

@@ -42,7 +42,7 @@ const TileEntry* find_tile(int prec, int L, bool loadT, bool storeT, int want_nb
     const TileEntry* first = nullptr;
     for (int i = 0; i < kNumTiles; ++i) {
         const TileEntry& e = kTiles[i];
-        if (e.prec == prec && e.L == L && e.loadT == loadT && e.storeT == storeT && !e.split) {
+        if (e.prec == prec && e.L == L && e.loadT == loadT && e.storeT == storeT) {
             if ((want_nb || want_wg) && (!want_nb || e.nb == want_nb) && (!want_wg || e.wg == want_wg)) return &e;
             if (!first) first = &e;
         }
@@ -50,12 +50,18 @@ const TileEntry* find_tile(int prec, int L, bool loadT, bool storeT, int want_nb
     return first;
 }
 
-// The fused-split variant of a T->T pass tile (same length and shape), or null.
-const TileEntry* find_split_tile(const TileEntry* k)
+// The variant of tile k (same length, shape, radices and access modes) with the real-transform
+// split `kind` (REAL_POST, REAL_PRE) fused in, or null.
+const TileEntry* find_real_tile(const TileEntry* k, int kind)
 {
-    for (int i = 0; i < kNumTiles; ++i) {
-        const TileEntry& e = kTiles[i];
-        if (e.split && e.prec == k->prec && e.L == k->L && e.nb == k->nb && e.wg == k->wg) return &e;
+    for (int i = 0; i < kNumRealTiles; ++i) {
+        const TileEntry& e = kRealTiles[i];
+        if (e.real != kind || e.prec != k->prec || e.L != k->L || e.nb != k->nb || e.wg != k->wg ||
+            e.loadT != k->loadT || e.storeT != k->storeT || e.nrad != k->nrad)
+            continue;
+        bool same = true;
+        for (int r = 0; r < e.nrad; ++r) same = same && e.radices[r] == k->radices[r];
+        if (same) return &e;
     }
     return nullptr;
 }
@@ -114,7 +120,7 @@ std::vector<std::vector<int>> candidate_passes(int prec, unsigned long long n, s
 {
     std::vector<int> lens;
     for (int i = 0; i < kNumTiles; ++i)
-        if (kTiles[i].prec == prec && kTiles[i].loadT && kTiles[i].storeT && !kTiles[i].split) lens.push_back(kTiles[i].L);
+        if (kTiles[i].prec == prec && kTiles[i].loadT && kTiles[i].storeT) lens.push_back(kTiles[i].L);
     std::sort(lens.begin(), lens.end());
     lens.erase(std::unique(lens.begin(), lens.end()), lens.end());
     std::vector<std::pair<double, std::vector<int>>> found;
@@ -286,19 +292,22 @@ int dev_alloc_list(std::vector<void*>& owned, void** out, size_t bytes)
     return GFFTX_SUCCESS;
 }
 
-int upload_stage_table_r(std::vector<void*>& owned, int prec, int nrad, const int* radices, void** out)
+int upload_stage_table_r(std::vector<void*>& owned, int prec, int nrad, const int* radices, void** out,
+                         double scale)
 {
     *out = nullptr;
     if (nrad <= 1) return GFFTX_SUCCESS;
     if (prec == PREC_F64) {
         std::vector<double> t;
         pack_stage_table(nrad, radices, t);
+        for (double& v : t) v *= scale;
         int rc = dev_alloc_list(owned, out, t.size() * sizeof(double));
         if (rc) return rc;
         HIP_TRY(hipMemcpy(*out, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
     } else {
         std::vector<float> t;
         pack_stage_table(nrad, radices, t);
+        for (float& v : t) v = (float)(v * scale);
         int rc = dev_alloc_list(owned, out, t.size() * sizeof(float));
         if (rc) return rc;
         HIP_TRY(hipMemcpy(*out, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));

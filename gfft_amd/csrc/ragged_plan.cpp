@@ -562,7 +562,8 @@ int gfftx_ragged_create(gfftx_ragged* plan, const uint64_t* lengths, uint64_t co
 }
 
 // Calls from several threads run one at a time; a call on another stream than the previous
-// call's waits for that call's work on the device (the side-stream scratch is shared).
+// call's waits until that call's work is done (the work-item counter and the side-stream
+// scratch are shared).
 int gfftx_ragged_exec(gfftx_ragged plan, void* data, void* stream)
 {
     if (!plan) return fail(GFFTX_ERR_INVALID_ARG, "plan is NULL");
@@ -572,11 +573,7 @@ int gfftx_ragged_exec(gfftx_ragged plan, void* data, void* stream)
     if (!g.ok) return fail(GFFTX_ERR_DEVICE, "cannot select device %d", plan->device);
     hipStream_t s = (hipStream_t)stream;
     HIP_TRY(plan->order.before(s));
-    const int rc = ragged_exec_any(plan, data, s);
-    const hipError_t oe = plan->order.after(s);
-    if (rc) return rc;
-    if (oe != hipSuccess) return fail(GFFTX_ERR_DEVICE, "stream order event: %s", hipGetErrorString(oe));
-    return GFFTX_SUCCESS;
+    return ragged_exec_any(plan, data, s);
 }
 
 int gfftx_ragged_describe(gfftx_ragged plan, char* buf, size_t len)
@@ -611,7 +608,6 @@ int gfftx_ragged_destroy(gfftx_ragged plan)
     if (!plan) return GFFTX_SUCCESS;
     DeviceGuard g(plan->device);
     plan->graphs.clear();
-    plan->order.clear();
     for (auto& lc : plan->longs)
         if (lc.blue) gfftx_plan_destroy(lc.blue);
     for (int k = 0; k < kRaggedSides; ++k) {

@@ -26,17 +26,21 @@ struct TileEntry {
     int lds_bytes;
     TileLaunch launch;
     const char* name;
-    bool split;  // SplitCfg: last pass of a real forward transform with the R2C split fused in
+    int real;  // REAL_NONE, or the real-transform split fused in (REAL_POST, REAL_PRE: RealCfg)
 };
 
+// Complex tiles (kernels.hip) and tiles with a real-transform split fused in (real_tiles.hip).
 extern TileEntry kTiles[];
 extern const int kNumTiles;
+extern TileEntry kRealTiles[];
+extern const int kNumRealTiles;
 
 // Split-precision MFMA kernels (FP32, lengths 16^S).
 typedef hipError_t (*MfmaLaunch)(const MfmaArgs&, dim3, hipStream_t, hipEvent_t, hipEvent_t);
 struct MfmaEntry {
     int L;
     int nb;
+    int scale;  // MFMA_SCALE_STAGE (stage twiddles carry 2^-4) or MFMA_SCALE_COLUMN
     MfmaLaunch launch;
     const char* name;
 };

@@ -43,7 +43,8 @@ struct TileArgs {
     double scale;
     const long long* seq_off;  // optional: element offset of every sequence (E-mode in/out)
     const long long* boff;     // optional: element offset of every batch item (else by*bdist)
-    // fused R2C split (SplitCfg tiles, last pass of a real forward transform)
+    // fused real-transform split (RealCfg tiles): the R2C post-pass in the last pass of a real
+    // forward transform, or the C2R pre-pass in the first pass of a real inverse transform
     const double2* sp_lo;      // two-level FP64 table of W_{2N}^k
     const double2* sp_hi;
     int sp_shift;
@@ -98,23 +99,36 @@ struct TileCfg {
     static constexpr bool DIRECT = (LOADT == STORET);
     static constexpr bool NEED_LDS = (NS > 1) || !DIRECT;
     static constexpr int LDS_ELEMS = NEED_LDS ? NB * TS : 1;
-    static constexpr bool SPLIT = false;
+    static constexpr int REAL = 0;          // REAL_NONE: no real-transform split in this tile
+    static constexpr bool PAIRED = false;   // lanes pair sequences p and S - p
+    static constexpr int RUN = LOADT ? NB : NB * L;  // contiguous elements per loaded row
     __device__ static constexpr int lidx(int t, int e) { return t * TS + e + (e >> 4); }
 };
 
-// Last pass of a real forward transform with the R2C split fused in.  The pass computes
-// Z[t + q S] for its S = ntrans sequences t (T->T tile); the split needs Z[k] and Z[N-k],
-// and N - (t + q S) = (S - t) + (L - 1 - q) S for t > 0 (sequence 0 pairs with itself,
-// element q with L - q).  So a tile takes H = NB/2 sequences p and their partners S - p
-// (lanes i and i + H); tile 0 takes the self-paired sequences 0 and, for even S, S/2.
-template <class Base> struct SplitCfg : Base {
-    static_assert(Base::LOADT && Base::STORET, "T->T tiles");
-    static_assert(Base::NB % 2 == 0 && Base::NB >= 2, "sequences pair up inside a tile");
-    static constexpr bool SPLIT = true;
-    static constexpr int LDS_ELEMS = Base::NB * Base::TS;  // the outputs meet in LDS
-};
+enum { REAL_NONE = 0, REAL_POST = 1, REAL_PRE = 2 };
 
-// Sequence of lane t in tile `tile` under the split pairing (SplitCfg), or -1 for none.
+// A tile with the real-transform split fused in.  REAL_POST: the R2C post-pass after the
+// last stage of a real forward transform (T->T last pass, or E->E single pass); REAL_PRE: the
+// C2R pre-pass before the first stage of a real inverse transform (T->E first pass, or E->E
+// single pass).  The split needs Z[k] and Z[N-k].  In an E->E tile a sequence is the whole
+// transform, so both are in the tile.  In a pass over S = ntrans sequences, element q of
+// sequence t is t + q S, and N - (t + q S) = (S - t) + (L - 1 - q) S for t > 0 (sequence 0
+// pairs with itself, element q with L - q): a T-mode tile then takes H = NB/2 sequences p
+// and their partners S - p (lanes i and i + H), and tile 0 the self-paired sequences 0 and,
+// for even S, S/2.
+template <class Base, int KIND> struct RealCfg : Base {
+    static_assert(KIND == REAL_POST || KIND == REAL_PRE, "split kind");
+    static_assert(KIND != REAL_POST || Base::LOADT == Base::STORET, "post-pass: T->T or E->E tiles");
+    static_assert(KIND != REAL_PRE || !Base::STORET, "pre-pass: T->E or E->E tiles");
+    static constexpr int REAL = KIND;
+    static constexpr bool PAIRED = Base::LOADT;
+    static_assert(!PAIRED || (Base::NB % 2 == 0 && Base::NB >= 2), "sequences pair up inside a tile");
+    static constexpr int RUN = PAIRED ? Base::NB / 2 : Base::RUN;
+    static constexpr int LDS_ELEMS = Base::NB * Base::TS;  // the two halves meet in LDS
+};
+template <class Base> using SplitCfg = RealCfg<Base, REAL_POST>;
+
+// Sequence of lane t in tile `tile` under the split pairing (PAIRED tiles), or -1 for none.
 template <class C> __device__ __forceinline__ long long split_seq(long long S, unsigned tile, int t)
 {
     constexpr int H = C::NB / 2;
@@ -195,7 +209,7 @@ template <class C, bool IDLE> __device__ __forceinline__ void load_first(const C
     constexpr int B = C::nbf(0);
     constexpr int LR = C::L / R;
     constexpr bool GUARD = (B * C::TPT > LR);
-    constexpr bool NT = (C::LOADT ? C::NB : C::NB * C::L) * (int)sizeof(V) % 128 == 0;
+    constexpr bool NT = C::RUN * (int)sizeof(V) % 128 == 0;
 #pragma unroll
     for (int b = 0; b < B; ++b) {
         const int j = x.jt + b * C::TPT;
@@ -270,7 +284,7 @@ template <class C, bool IDLE, int S> __device__ __forceinline__ void tile_stage(
 
     if constexpr (!FIRST) __syncthreads();
 
-    if constexpr (LAST && C::DIRECT && !C::SPLIT) {
+    if constexpr (LAST && C::DIRECT && C::REAL != REAL_POST) {
         if (x.valid && (!IDLE || x.active)) {
             const TileArgs& a = x.a;
             const unsigned mask = a.conj_out ? 0x80000000u : 0u;
@@ -351,7 +365,7 @@ __device__ __forceinline__ Ctx<C> make_ctx(const TileArgs& a, unsigned tile, uns
         x.jt = 0;
     }
     long long tl;
-    if constexpr (C::SPLIT) {
+    if constexpr (C::PAIRED) {
         x.tg = split_seq<C>(a.ntrans, tile, x.t);
         x.valid = x.tg >= 0;
         tl = x.valid ? x.tg : 0;
@@ -386,6 +400,24 @@ template <class C, bool IDLE> __device__ __forceinline__ void tile_finish(const 
         const bool scale = a.scale != 1.0;
         const int tid = threadIdx.x;
         V* out = reinterpret_cast<V*>(a.out) + x.bo;
+        if constexpr (C::PAIRED) {
+            // paired sequences p and S - p (the C2R pre-pass of a first pass): element lin of
+            // the tile is element q = lin % L of lane lin / L's sequence, stored by the rule
+            if (!IDLE || x.active) {
+#pragma unroll
+                for (int i = 0; i < PER; ++i) {
+                    const int lin = tid + i * C::WG;
+                    const int ts = lin / C::L, q = lin % C::L;
+                    const long long tgs = split_seq<C>(a.ntrans, tile, ts);
+                    if (tgs >= 0) {
+                        V u = x.lds[C::lidx(ts, q)];
+                        if (scale) u = cscale(u, sc);
+                        out[out_base<C>(a, tgs) + (unsigned)q * ns] = flip_im(u, mask);
+                    }
+                }
+            }
+            return;
+        }
         if (!C::STORET && ns == 1 && a.seq_off == nullptr) {
             // Contiguous sequences (the first pass of a multi-pass transform, batched E-mode
             // output): the tile's NB*L outputs form one contiguous run, and element lin of the
@@ -506,6 +538,132 @@ template <class C, bool IDLE> __device__ __forceinline__ void split_finish(const
     }
 }
 
+// R2C split of an E->E tile (REAL_POST, single pass): the LDS holds each sequence's Z in
+// natural order and the sequence is the whole transform, so X[k] takes Z[k] and Z[N-k] from
+// the same sequence.  Thread (t, i0) writes the elements k = i0 + i*TPT of sequence t; their
+// twiddles W_{2N}^k step by W_{2N}^TPT (FP64 recurrence from two table roots).
+template <class C, bool IDLE> __device__ __forceinline__ void split_finish_e(const Ctx<C>& x)
+{
+    using V = typename C::V;
+    using S_ = decltype(V{}.x);
+    constexpr int L = C::L, PER = L / C::TPT;
+    static_assert(L % C::TPT == 0, "whole elements per thread");
+    const TileArgs& a = x.a;
+    if ((IDLE && !x.active) || !x.valid) return;
+    const S_ half = (S_)0.5;
+    V* out = x.out;
+    const int i0 = x.jt;
+    const long long mask = (1LL << a.sp_shift) - 1;
+    double2 w = dmul(a.sp_lo[i0 & mask], a.sp_hi[i0 >> a.sp_shift]);
+    const double2 step = dmul(a.sp_lo[C::TPT & mask], a.sp_hi[C::TPT >> a.sp_shift]);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int k = i0 + i * C::TPT;
+        const V zk = x.lds[C::lidx(x.t, k)], zp = x.lds[C::lidx(x.t, k == 0 ? 0 : L - k)];
+        if (k == 0) {
+            const S_ x0 = zk.x + zk.y, xn = zk.x - zk.y;
+            if (a.split_bins) {
+                out[0] = V{x0, 0};
+                out[L] = V{xn, 0};
+            } else {
+                out[0] = V{x0, xn};
+            }
+        } else {
+            const bool lo = k <= L - k;
+            const V A = lo ? zk : zp, B = lo ? zp : zk;
+            const V wv = to_v<V>(lo ? w : double2{-w.x, w.y});  // W^k, or W^{N-k} = -conj(W^k)
+            const V h1 = V{half * (A.x + B.x), half * (A.y - B.y)};
+            const V dd = V{half * (A.x - B.x), half * (A.y + B.y)};
+            const V p = cmul(wv, V{dd.y, -dd.x});
+            out[k] = lo ? V{h1.x + p.x, h1.y + p.y} : V{h1.x - p.x, p.y - h1.y};
+        }
+        w = dmul(w, step);
+    }
+}
+
+// C2R pre-pass of a REAL_PRE tile, between the loads and the first stage: every loaded X[e]
+// goes to LDS, each thread reads the partner X[N-e] of each of its elements and replaces
+// X[e] by Z[e], the input of the inverse complex transform:
+//   h1 = (A + conj B)/2,  d = (A - conj B)/2,  Z[k] = h1 + i conj(W_{2N}^k) d,
+//   Z[N-k] = conj(h1 - i conj(W_{2N}^k) d)      (A = X[k], B = X[N-k], k <= N-k)
+//   Z[0] = ((X_0 + X_N)/2, (X_0 - X_N)/2)
+// with X_N in the imaginary part of element 0 (NR packing) or at element N (bins).  These are
+// the relations of the split kernel's inverse branch.  E->E tiles: the partner is element
+// N-e of the same sequence.  Paired T->E tiles (first pass, element q of sequence t is
+// e = t + q S): the partner of (t, q) is (S - t, L - 1 - q), or (0, L - q) for t = 0.
+template <class C, bool IDLE> __device__ __forceinline__ void pre_split(const Ctx<C>& x, FirstIn<C>& f)
+{
+    using V = typename C::V;
+    using S_ = decltype(V{}.x);
+    constexpr int R = C::R[0];
+    constexpr int B = C::nbf(0);
+    constexpr int LR = C::L / R;
+    constexpr int H = C::NB / 2;
+    constexpr bool GUARD = (B * C::TPT > LR);
+    const TileArgs& a = x.a;
+    const bool act = !IDLE || x.active;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int j = x.jt + b * C::TPT;
+        if (!act || (GUARD && j >= LR)) continue;
+#pragma unroll
+        for (int r = 0; r < R; ++r) x.lds[C::lidx(x.t, j + r * LR)] = f.v[b][r];
+    }
+    __syncthreads();
+    if (act && x.valid) {
+        const S_ half = (S_)0.5;
+        const long long N = a.nfull;
+        // lane of the partner sequence, the sequence's first element e0 and the element step
+        int pl = x.t;
+        long long e0 = 0, es = 1;
+        if constexpr (C::PAIRED) {
+            const unsigned tile_lane = (unsigned)x.t;
+            pl = (x.tg == 0 || 2 * x.tg == a.ntrans) ? x.t : (tile_lane < (unsigned)H ? x.t + H : x.t - H);
+            e0 = x.tg;
+            es = a.ntrans;
+        }
+        const long long mask = (1LL << a.sp_shift) - 1;
+        const long long rst = (long long)LR * es;  // e steps by LR S between a thread's r
+        const double2 step = dmul(a.sp_lo[rst & mask], a.sp_hi[rst >> a.sp_shift]);
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int j = x.jt + b * C::TPT;
+            if (GUARD && j >= LR) continue;
+            const long long eb = e0 + (long long)j * es;
+            double2 w = dmul(a.sp_lo[eb & mask], a.sp_hi[eb >> a.sp_shift]);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int q = j + r * LR;
+                const long long e = eb + (long long)r * rst;
+                int qp;
+                if constexpr (C::PAIRED) qp = x.tg == 0 ? (q == 0 ? 0 : C::L - q) : C::L - 1 - q;
+                else qp = q == 0 ? 0 : C::L - q;
+                const V xe = f.v[b][r], xp = x.lds[C::lidx(pl, qp)];
+                V z;
+                if (e == 0) {
+                    const S_ x0 = xe.x;
+                    // e == 0 is element 0 of sequence 0, so x.in is the item's start
+                    const S_ xn = a.split_bins ? x.in[N].x : xe.y;
+                    z = V{half * (x0 + xn), half * (x0 - xn)};
+                } else {
+                    const bool lo = e <= N - e;
+                    const V A = lo ? xe : xp, Bv = lo ? xp : xe;
+                    // conj(W^k) for k = min(e, N - e): conj(W^{N-e}) = -W^e
+                    const V cw = to_v<V>(lo ? double2{w.x, -w.y} : double2{-w.x, -w.y});
+                    const V h1 = V{half * (A.x + Bv.x), half * (A.y - Bv.y)};
+                    const V dd = V{half * (A.x - Bv.x), half * (A.y + Bv.y)};
+                    const V t = cmul(cw, dd);
+                    const V p = V{-t.y, t.x};  // i conj(W^k) d
+                    z = lo ? V{h1.x + p.x, h1.y + p.y} : V{h1.x - p.x, p.y - h1.y};
+                }
+                f.v[b][r] = z;
+                w = dmul(w, step);
+            }
+        }
+    }
+    __syncthreads();  // the first stage writes the LDS next
+}
+
 // One tile, loaded and transformed.  Callable from any kernel whose workgroup has at least
 // C::WG threads (IDLE: more than C::WG, the extra ones only join the barriers) and which
 // provides C::LDS_ELEMS elements of LDS.
@@ -515,8 +673,10 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, unsigned tile, unsi
     Ctx<C> x = make_ctx<C, IDLE>(a, tile, by, lds);
     FirstIn<C> f;
     load_first<C, IDLE>(x, f);
+    if constexpr (C::REAL == REAL_PRE) pre_split<C, IDLE>(x, f);
     stage_loop<C, IDLE, 0>(x, f);
-    if constexpr (C::SPLIT) split_finish<C, IDLE>(x, tile);
+    if constexpr (C::REAL == REAL_POST && C::PAIRED) split_finish<C, IDLE>(x, tile);
+    else if constexpr (C::REAL == REAL_POST) split_finish_e<C, IDLE>(x);
     else tile_finish<C, IDLE>(x, tile);
 }
 

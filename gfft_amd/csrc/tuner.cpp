@@ -160,10 +160,12 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
     if ((p->passes.size() < 2 && !single_pass) || (flags & GFFTX_FLAG_NO_AUTOTUNE) ||
         env_int("GFFTX_AUTOTUNE", 1) == 0 || getenv("GFFTX_PASSES"))
         return GFFTX_SUCCESS;
-    // a real forward plan is timed with the R2C split fused into its last pass, and every
-    // candidate runs with the plan's own placement (one buffer or two), so the choice is kept
-    // per transform kind and place; forward and inverse complex transforms share it
-    const std::string key = tune_key(device, p->prec, n, batch) + (ttype == GFFTX_RDFT ? "|rdft" : "") +
+    // a real plan is timed with the split fused into its last (forward) or first (inverse)
+    // pass, and every candidate runs with the plan's own placement (one buffer or two), so
+    // the choice is kept per transform kind and place; forward and inverse complex transforms
+    // share it
+    const std::string key = tune_key(device, p->prec, n, batch) +
+                            (ttype == GFFTX_RDFT ? "|rdft" : ttype == GFFTX_IRDFT ? "|irdft" : "") +
                             (place == GFFTX_IN_PLACE ? "|ip" : "|oop");
     std::vector<int> chosen;
     std::string note;
@@ -194,7 +196,7 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
                 const TileEntry* def = find_tile(p->prec, L, true, later);
                 for (int t = 0; t < kNumTiles; ++t) {
                     const TileEntry& e = kTiles[t];
-                    if (e.prec != p->prec || e.L != L || !e.loadT || e.storeT != later || e.split || &e == def) continue;
+                    if (e.prec != p->prec || e.L != L || !e.loadT || e.storeT != later || &e == def) continue;
                     std::vector<int> c = sp;
                     c.push_back(0);
                     for (size_t j = 0; j < sp.size(); ++j) {
@@ -209,7 +211,7 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
         if (single_pass)
             for (int t = 0; t < kNumTiles; ++t) {
                 const TileEntry& e = kTiles[t];
-                if (e.prec == p->prec && (uint64_t)e.L == n && !e.loadT && !e.storeT && !e.split)
+                if (e.prec == p->prec && (uint64_t)e.L == n && !e.loadT && !e.storeT)
                     cands.push_back({(int)n, 1, 0, e.nb, e.wg});
             }
         if (cands.size() < 2) return GFFTX_SUCCESS;
@@ -344,7 +346,7 @@ bool tune_length_class(int device, int prec, long long L, long long count, bool 
         std::vector<std::vector<int>> cands;
         for (int t = 0; has_single && t < kNumTiles; ++t) {
             const TileEntry& e = kTiles[t];
-            if (e.prec == prec && e.L == L && !e.loadT && !e.storeT && !e.split) cands.push_back({(int)L, 1, 0, e.nb, e.wg});
+            if (e.prec == prec && e.L == L && !e.loadT && !e.storeT) cands.push_back({(int)L, 1, 0, e.nb, e.wg});
         }
         for (const std::vector<int>& sp : splits) cands.push_back(sp);
         DeviceGuard g(device);

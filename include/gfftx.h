@@ -35,10 +35,12 @@
  * chunk, default 16).  `stream` is a hipStream_t; NULL is the null stream.  Calls on one plan
  * may come from several threads (GFFT's fft() is reentrant): the plan runs them one at a
  * time, a host-pointer call until its result is back, a device-pointer call until its work
- * is enqueued.  Calls on one plan from different streams are ordered on the device: a call
- * on another stream than the previous call's first waits for that call's work (the plan's
- * scratch arrays serve every call).  A call on a stream the caller is capturing into a graph
- * is not ordered that way; the caller orders the replays of its own graphs.
+ * is enqueued.  Calls on one plan from different streams are safe: when the plan's calls
+ * share device arrays (multi-pass scratch, chirp-z work arrays, graphs, ragged batches), a
+ * call on another stream than the previous call's first waits until the device has finished
+ * its earlier work (a host wait, only at such a switch; calls on one stream cost nothing
+ * extra).  A call on a stream the caller is capturing into a graph is not ordered that way;
+ * the caller orders the replays of its own graphs.
  * No function throws; each returns a gfftx_status.  gfftx_last_error() gives detail for the
  * calling thread.
  */

@@ -22,11 +22,18 @@ def host(t):
     return t.cpu().numpy()
 
 
+SCALES = ["stage", "column"]
+
+
+@pytest.mark.parametrize("scale", SCALES)
 @pytest.mark.parametrize("n,batch", [(256, 64), (4096, 16)])
-def test_mfma_forward_inverse(torch_cuda, n, batch):
+def test_mfma_forward_inverse(torch_cuda, monkeypatch, scale, n, batch):
+    """Both scalings of the FP16 split (per stage, the default; per column) against the
+    long-double oracle, forward in place and inverse out of place."""
+    monkeypatch.setenv("GFFTX_MFMA_SCALE", scale)
     x = inputs.complex_normal(n * batch, 500 + n).astype(np.complex64).reshape(batch, n)
     with api.Plan(n, api.COMPLEX_FLOAT, api.DFT, api.IN_PLACE, batch=batch, flags=api.FLAG_SPLIT_MFMA) as p:
-        assert "mfma" in p.describe()
+        assert "mfma" in p.describe() and ("_col" in p.describe()) == (scale == "column")
         d = dev(torch_cuda, x)
         p.fft(d)
         y = host(d)
@@ -41,8 +48,11 @@ def test_mfma_forward_inverse(torch_cuda, n, batch):
         assert rel_l2(yi[b], O.ref_fft(xb, -1) / n) <= 2e-7
 
 
-def test_mfma_scaling_edge_cases(torch_cuda):
-    n, batch = 4096, 6
+@pytest.mark.parametrize("scale", SCALES)
+@pytest.mark.parametrize("n", [4096, 256])
+def test_mfma_scaling_edge_cases(torch_cuda, monkeypatch, scale, n):
+    monkeypatch.setenv("GFFTX_MFMA_SCALE", scale)
+    batch = 6
     x = inputs.complex_normal(n * batch, 9).astype(np.complex64).reshape(batch, n)
     x[0] = 0                        # all-zero transform (every column max is 0)
     x[1] *= 1e-20                   # tiny magnitudes
@@ -89,3 +99,9 @@ def test_c5_config_16384x4096(torch_cuda):
     print(f"C5 sampled worst relL2: mfma {worst_m:.3e}, plain {worst_p:.3e}")
     assert worst_m <= 2e-7 and worst_p <= 2e-7
     assert rel_l2(ym, yp) <= 3e-7
+    # every transform against numpy's complex128 FFT (pocketfft, an independent FP64 reference)
+    ref = np.fft.fft(x.astype(np.complex128), axis=-1)
+    rm = np.linalg.norm(ym - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    rp = np.linalg.norm(yp - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    print(f"C5 all 16384 transforms, worst relL2 vs numpy: mfma {rm.max():.3e}, plain {rp.max():.3e}")
+    assert rm.max() <= 2e-7 and rp.max() <= 2e-7

@@ -250,3 +250,96 @@ def test_real_every_even_length_to_256(torch_cuda, dtype):
             pi.fft(d)
             assert rel_l2(host(d), z) <= 2 * tol, n
     print(f"real every even length 2..256 {np.dtype(dtype).name}: worst relL2 {worst[0]:.2e} at n={worst[1]}")
+
+
+@pytest.mark.parametrize("dtype,n,passes,batch,layout", [
+    (np.complex64, 2 ** 15, "128,256", 3, "nr"), (np.complex64, 49152, "256,192", 1, "bins"),
+    (np.complex64, 46080, "1024,45", 2, "nr"), (np.complex64, 2 ** 18, "256,1024", 1, "bins"),
+    (np.complex64, 3 * 2 ** 19, "1536,1024", 1, "bins"), (np.complex64, 3 * 2 ** 19, "1536,1024", 2, "nr"),
+    (np.complex64, 20480, "4096,5", 1, "nr"), (np.complex64, 2 ** 20, "1024,1024", 1, "nr"),
+    (np.complex128, 2 ** 14, "64,256", 2, "nr"), (np.complex128, 40960, "320,128", 1, "bins"),
+    (np.complex128, 46080, "1024,45", 1, "nr"), (np.complex128, 12288, "4096,3", 1, "nr"),
+    (np.complex128, 2 ** 20, "256,16,256", 1, "bins")])
+def test_fused_presplit_first_pass(torch_cuda, monkeypatch, dtype, n, passes, batch, layout):
+    """Multi-pass real inverse transforms fuse the C2R pre-pass into the first pass: its tiles
+    pair sequence t with S - t, S = n / (first pass length), even and odd (5, 45, 3).  The
+    result must return the input of the forward transform (the spectrum comes from the
+    long-double oracle) and agree with the separate split kernel; out of place the input is
+    left unchanged, and in place gives the same result."""
+    monkeypatch.setenv("GFFTX_PASSES", passes)
+    z = inputs.complex_normal(n * batch, 500 + n).astype(dtype).reshape(batch, n)
+    flags = api.FLAG_R2C_BINS if layout == "bins" else 0
+    spec = []
+    for b in range(batch):
+        ref, X = packed_ref(z[b])
+        spec.append(X if layout == "bins" else ref)
+    spec = np.stack(spec).astype(dtype)
+    print(f"case fused presplit n={n} batch={batch} {layout} {np.dtype(dtype).name}", flush=True)
+    outs = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("GFFTX_FUSED_SPLIT", fused)
+        with api.Plan(n, VT[dtype], api.IRDFT, api.OUT_OF_PLACE, batch=batch, flags=flags) as p:
+            assert ("_presplit" in p.describe()) == (fused == "1")
+            s = dev(torch_cuda, spec)
+            o = torch_cuda.zeros((batch, n), dtype=s.dtype, device="cuda")
+            p.fft(s, o)
+            outs[fused] = host(o)
+            assert np.array_equal(host(s), spec)
+    for b in range(batch):
+        assert rel_l2(outs["1"][b], z[b]) <= 2 * TOL[dtype], b
+    assert rel_l2(outs["1"], outs["0"]) <= 2 * TOL[dtype]
+    if layout == "nr":
+        monkeypatch.setenv("GFFTX_FUSED_SPLIT", "1")
+        with api.Plan(n, VT[dtype], api.IRDFT, api.IN_PLACE, batch=batch) as p:
+            d = dev(torch_cuda, spec)
+            p.fft(d)
+            assert rel_l2(host(d), outs["1"]) <= TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n,batch,layout", [(2, 5, "nr"), (16, 3, "nr"), (64, 1, "bins"), (256, 7, "nr"),
+                                            (96, 4, "nr"), (1280, 2, "nr"), (2048, 1, "bins"), (4096, 3, "nr"),
+                                            (6144, 1, "bins"), (8192, 2, "nr")])
+def test_fused_split_single_pass(torch_cuda, monkeypatch, dtype, n, batch, layout):
+    """Single-pass real transforms run the R2C post-pass and the C2R pre-pass inside the
+    batched tile (one kernel per call): forward against the long-double packed reference,
+    inverse back to the input, both against the separate split kernel, in and out of place."""
+    z = inputs.complex_normal(n * batch, 700 + n).astype(dtype).reshape(batch, n)
+    flags = api.FLAG_R2C_BINS if layout == "bins" else 0
+    w = n + 1 if layout == "bins" else n
+    refs = [packed_ref(z[b]) for b in range(batch)]
+    want = np.stack([X if layout == "bins" else ref for ref, X in refs])
+    print(f"case fused single-pass n={n} batch={batch} {layout} {np.dtype(dtype).name}", flush=True)
+    fwd, inv = {}, {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("GFFTX_FUSED_SPLIT", fused)
+        with api.Plan(n, VT[dtype], api.RDFT, api.OUT_OF_PLACE, batch=batch, flags=flags) as pf, \
+                api.Plan(n, VT[dtype], api.IRDFT, api.OUT_OF_PLACE, batch=batch, flags=flags) as pi:
+            passes_f = json.loads(pf.describe())["passes"]
+            passes_i = json.loads(pi.describe())["passes"]
+            assert len(passes_f) == 1 and len(passes_i) == 1
+            assert ("_e_split" in passes_f[0]["kernel"]) == (fused == "1")
+            assert ("_e_presplit" in passes_i[0]["kernel"]) == (fused == "1")
+            s = dev(torch_cuda, z)
+            o = torch_cuda.zeros((batch, w), dtype=s.dtype, device="cuda")
+            pf.fft(s, o)
+            fwd[fused] = host(o)
+            sp = dev(torch_cuda, want.astype(dtype))
+            b = torch_cuda.zeros((batch, n), dtype=s.dtype, device="cuda")
+            pi.fft(sp, b)
+            inv[fused] = host(b)
+            assert np.array_equal(host(sp), want.astype(dtype))
+    for k in range(batch):
+        assert rel_l2(fwd["1"][k], want[k]) <= TOL[dtype], k
+        assert rel_l2(inv["1"][k], z[k]) <= 2 * TOL[dtype], k
+    assert rel_l2(fwd["1"], fwd["0"]) <= TOL[dtype]
+    assert rel_l2(inv["1"], inv["0"]) <= 2 * TOL[dtype]
+    if layout == "nr":
+        monkeypatch.setenv("GFFTX_FUSED_SPLIT", "1")
+        with api.Plan(n, VT[dtype], api.RDFT, api.IN_PLACE, batch=batch) as pf, \
+                api.Plan(n, VT[dtype], api.IRDFT, api.IN_PLACE, batch=batch) as pi:
+            d = dev(torch_cuda, z)
+            pf.fft(d)
+            assert rel_l2(host(d), fwd["1"]) <= TOL[dtype]
+            pi.fft(d)
+            assert rel_l2(host(d), z) <= 2 * TOL[dtype]
