@@ -184,10 +184,10 @@ static hipError_t launch_mfma(const MfmaArgs& a, dim3 grid, hipStream_t s, hipEv
 // Per-stage scaling first (the default); the per-column variants remain for comparison
 // (GFFTX_MFMA_SCALE=column).
 MfmaEntry kMfma[] = {
-    {256, 16, MFMA_SCALE_STAGE, &launch_mfma<MfmaCfg<256, 16>>, "mfma_split_f32_L256_nb16"},
-    {4096, 1, MFMA_SCALE_STAGE, &launch_mfma<MfmaCfg<4096, 1>>, "mfma_split_f32_L4096_nb1"},
-    {256, 16, MFMA_SCALE_COLUMN, &launch_mfma<MfmaCfg<256, 16, MFMA_SCALE_COLUMN>>, "mfma_split_f32_L256_nb16_col"},
-    {4096, 1, MFMA_SCALE_COLUMN, &launch_mfma<MfmaCfg<4096, 1, MFMA_SCALE_COLUMN>>, "mfma_split_f32_L4096_nb1_col"},
+    {256, 16, MFMA_SCALE_STAGE, 256, &launch_mfma<MfmaCfg<256, 16>>, "mfma_split_f32_L256_nb16"},
+    {4096, 1, MFMA_SCALE_STAGE, 256, &launch_mfma<MfmaCfg<4096, 1>>, "mfma_split_f32_L4096_nb1"},
+    {256, 16, MFMA_SCALE_COLUMN, 256, &launch_mfma<MfmaCfg<256, 16, MFMA_SCALE_COLUMN>>, "mfma_split_f32_L256_nb16_col"},
+    {4096, 1, MFMA_SCALE_COLUMN, 256, &launch_mfma<MfmaCfg<4096, 1, MFMA_SCALE_COLUMN>>, "mfma_split_f32_L4096_nb1_col"},
 };
 const int kNumMfma = sizeof(kMfma) / sizeof(kMfma[0]);
 

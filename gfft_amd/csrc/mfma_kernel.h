@@ -40,15 +40,16 @@ constexpr int MFMA_A_HALFS = 2 * 2 * 64 * 8;
 
 enum { MFMA_SCALE_STAGE = 0, MFMA_SCALE_COLUMN = 1 };
 
-template <int L_, int NB_, int SCALE_ = MFMA_SCALE_STAGE> struct MfmaCfg {
-    static constexpr int L = L_, NB = NB_, WG = 256;
+template <int L_, int NB_, int SCALE_ = MFMA_SCALE_STAGE, int WG_ = 256> struct MfmaCfg {
+    static constexpr int L = L_, NB = NB_, WG = WG_;
     static constexpr int SCALE = SCALE_;
+    static constexpr int WAVES = WG / 64;
     static constexpr int COLS = L / 16;                 // columns per sequence and stage
     static constexpr int BLOCKS = NB * COLS / 16;       // 16-column blocks per stage (all sequences)
-    static constexpr int BPW = BLOCKS / 4;              // blocks per wave
+    static constexpr int BPW = BLOCKS / WAVES;          // blocks per wave
     static constexpr int NS = (L == 256) ? 2 : (L == 4096) ? 3 : (L == 65536) ? 4 : 0;
     static_assert(NS > 0, "length must be 16^S");
-    static_assert(BLOCKS % 4 == 0, "four waves share the blocks evenly");
+    static_assert(BLOCKS % WAVES == 0 && BPW >= 1, "the waves share the blocks evenly");
     static constexpr int LP = L + L / 16;
     static constexpr int TS = (LP % 2 == 0) ? LP + 1 : LP;
     static constexpr int LDS_ELEMS = NB * TS;
@@ -313,7 +314,7 @@ __device__ __forceinline__ void mfma_loop(const MfmaArgs& a, float2* lds, const 
     }
 }
 
-template <class C> __global__ __launch_bounds__(256) void mfma_fft_kernel(MfmaArgs a)
+template <class C> __global__ __launch_bounds__(C::WG) void mfma_fft_kernel(MfmaArgs a)
 {
     __shared__ float2 lds[C::LDS_ELEMS];
     const int lane = threadIdx.x & 63;

@@ -65,7 +65,7 @@ int build_c2c_passes(gfftx_plan_s* p)
         const MfmaEntry* me = nullptr;
         const char* sm = getenv("GFFTX_MFMA_SCALE");
         const int want = (sm && !strcmp(sm, "column")) ? MFMA_SCALE_COLUMN : MFMA_SCALE_STAGE;
-        for (int i = 0; i < kNumMfma; ++i)
+        for (int i = 0; i < kNumMfma && !me; ++i)
             if ((unsigned long long)kMfma[i].L == n && kMfma[i].scale == want) me = &kMfma[i];
         if (p->prec != PREC_F32 || p->real || !me)
             return fail(GFFTX_ERR_UNSUPPORTED, "split-precision MFMA path: FP32 C2C of length 256 or 4096 only (n=%llu)", n);

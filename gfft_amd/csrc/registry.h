@@ -41,6 +41,7 @@ struct MfmaEntry {
     int L;
     int nb;
     int scale;  // MFMA_SCALE_STAGE (stage twiddles carry 2^-4) or MFMA_SCALE_COLUMN
+    int wg;     // threads per workgroup
     MfmaLaunch launch;
     const char* name;
 };

@@ -104,8 +104,10 @@ static void SUF(bfp_transform)(T* d, long K, long MS, const TT* c, const TT* s,
             const long k = (kk > H) ? K - kk - 1 : kk - 1;
             const T s1 = (T)(s[k] * di[j]);
             const T s2 = (T)(s[k] * dr[j]);
-            re1 += (T)(c[k] * sr[j]);
-            im1 += (T)(c[k] * si[j]);
+            /* the product stays in TempType and the sum is rounded to T, as GFFT's
+               `re1 += m_c[k]*sr[j]` with re1 of type T (src/gfftspec_inp.h:63-64) */
+            re1 += c[k] * sr[j];
+            im1 += c[k] * si[j];
             re2 += sign_change ? -s1 : s1;
             im2 -= sign_change ? -s2 : s2;
         }
@@ -338,8 +340,8 @@ static void SUF(intime_oop)(const T* src, T* dst, long N, const long* fac, int n
                 const long kx = (kk > H) ? K - kk - 1 : kk - 1;
                 const T s1 = (T)(tw.s[kx] * di[j]);
                 const T s2 = (T)(tw.s[kx] * dr[j]);
-                re1 += (T)(tw.c[kx] * sr[j]);
-                im1 += (T)(tw.c[kx] * si[j]);
+                re1 += tw.c[kx] * sr[j];  /* TempType product and sum, rounded to T */
+                im1 += tw.c[kx] * si[j];
                 re2 += sign_change ? -s1 : s1;
                 im2 -= sign_change ? -s2 : s2;
             }
