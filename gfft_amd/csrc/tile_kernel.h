@@ -102,8 +102,26 @@ struct TileCfg {
     static constexpr int REAL = 0;          // REAL_NONE: no real-transform split in this tile
     static constexpr bool PAIRED = false;   // lanes pair sequences p and S - p
     static constexpr int RUN = LOADT ? NB : NB * L;  // contiguous elements per loaded row
+    // E->E tiles whose TPT divides the wave size: every sequence is worked on by the lanes of
+    // one wave, so the exchanges between stages need no workgroup barrier
+    static constexpr bool WAVE_LOCAL = !LOADT && !STORET && (64 % TPT == 0);
     __device__ static constexpr int lidx(int t, int e) { return t * TS + e + (e >> 4); }
 };
+
+// Orders the LDS accesses of the stages before it against those after it, for the threads
+// that share a sequence.  A wave's LDS instructions complete in program order, so when a
+// sequence lies in one wave a compiler-level fence suffices and the waves of a workgroup run
+// their stages without waiting for each other; otherwise the workgroup barrier.
+template <class C> __device__ __forceinline__ void stage_sync()
+{
+    if constexpr (C::WAVE_LOCAL) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
 
 enum { REAL_NONE = 0, REAL_POST = 1, REAL_PRE = 2 };
 
@@ -282,7 +300,7 @@ template <class C, bool IDLE, int S> __device__ __forceinline__ void tile_stage(
         Dft<R>::run(v[b]);
     }
 
-    if constexpr (!FIRST) __syncthreads();
+    if constexpr (!FIRST) stage_sync<C>();
 
     if constexpr (LAST && C::DIRECT && C::REAL != REAL_POST) {
         if (x.valid && (!IDLE || x.active)) {
@@ -318,7 +336,8 @@ template <class C, bool IDLE, int S> __device__ __forceinline__ void tile_stage(
 #pragma unroll
             for (int r = 0; r < R; ++r) x.lds[C::lidx(x.t, base + r * NSS)] = v[b][r];
         }
-        __syncthreads();
+        // after the last stage an E->E tile reads only its own sequence back (split_finish_e)
+        stage_sync<C>();
     }
 }
 
