@@ -150,6 +150,19 @@ TileEntry kTiles[] = {
     GFFTX_PASS_SMALL(PREC_F32, float),
     GFFTX_BATCH_7(PREC_F32, float),
     GFFTX_PASS_7(PREC_F32, float),
+    // pass tiles of 192, 320 and 80 points whose rows are whole 128-byte lines (16, 32 or 48
+    // sequences), so that their inputs load non-temporally; the autotuner weighs them against
+    // the default shapes above
+    GFFTX_TILE(PREC_F32, float, 192, 16, 192, true, false, 16, 4, 3),
+    GFFTX_TILE(PREC_F32, float, 192, 16, 192, true, true, 16, 4, 3),
+    GFFTX_TILE(PREC_F32, float, 192, 32, 384, true, false, 16, 4, 3),
+    GFFTX_TILE(PREC_F32, float, 192, 32, 384, true, true, 16, 4, 3),
+    GFFTX_TILE(PREC_F32, float, 320, 16, 320, true, false, 16, 4, 5),
+    GFFTX_TILE(PREC_F32, float, 320, 16, 320, true, true, 16, 4, 5),
+    GFFTX_TILE(PREC_F32, float, 80, 32, 160, true, false, 16, 5),
+    GFFTX_TILE(PREC_F32, float, 80, 32, 160, true, true, 16, 5),
+    GFFTX_TILE(PREC_F32, float, 80, 48, 240, true, false, 16, 5),
+    GFFTX_TILE(PREC_F32, float, 80, 48, 240, true, true, 16, 5),
     GFFTX_BATCH_POW2(PREC_F64, double),
     GFFTX_BATCH_MIXED(PREC_F64, double),
     GFFTX_PASS_SET(PREC_F64, double, 2, 512, 4, 256, 4, 384),

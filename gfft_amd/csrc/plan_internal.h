@@ -329,7 +329,7 @@ int exec_host(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, bool di
 int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttype, int place, int device,
                 uint32_t flags);
 bool tune_length_class(int device, int prec, long long L, long long count, bool inverse, bool* single,
-                       std::vector<int>* lens, int* nb, int* wg);
+                       std::vector<int>* lens, std::vector<std::pair<int, int>>* tiles);
 
 }  // namespace detail
 }  // namespace gfftx

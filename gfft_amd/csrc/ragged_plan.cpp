@@ -57,23 +57,25 @@ int ragged_long_class(gfftx_ragged_s* r, long long L, const std::vector<long lon
     std::vector<int> lens;
     bool multi = choose_passes(r->prec, (unsigned long long)L, lens, (unsigned long long)lc.count);
     const TileEntry* k1 = (L <= (1 << 30)) ? find_tile(r->prec, (int)L, false, false) : nullptr;
-    int t_nb = 0, t_wg = 0;
     bool use_single = k1 && (!multi || lc.count >= min_single);
-    // measured choice between the single-pass tile and the pass splits (unless the caller
-    // turned autotuning off or fixed the rule with GFFTX_RAGGED_MIN_SINGLE)
+    // measured choice between the single-pass tile and the pass splits with their tile shapes
+    // (unless the caller turned autotuning off or fixed the rule with GFFTX_RAGGED_MIN_SINGLE)
     bool t_single = false;
     std::vector<int> t_lens;
+    std::vector<std::pair<int, int>> t_tiles;  // (nb, wg) per pass, (0, 0) = the default tile
     if (!(r->flags & GFFTX_FLAG_NO_AUTOTUNE) && !getenv("GFFTX_RAGGED_MIN_SINGLE") && !getenv("GFFTX_PASSES") &&
-        tune_length_class(r->device, r->prec, L, lc.count, r->inverse, &t_single, &t_lens, &t_nb, &t_wg)) {
+        tune_length_class(r->device, r->prec, L, lc.count, r->inverse, &t_single, &t_lens, &t_tiles)) {
         if (t_single && k1) {
             use_single = true;
-            k1 = find_tile(r->prec, (int)L, false, false, t_nb, t_wg);
+            k1 = find_tile(r->prec, (int)L, false, false, t_tiles.empty() ? 0 : t_tiles[0].first,
+                           t_tiles.empty() ? 0 : t_tiles[0].second);
         } else if (!t_single && t_lens.size() >= 2) {
             use_single = false;
             multi = true;
             lens = t_lens;
         }
     }
+    if (!multi || use_single || lens != t_lens) t_tiles.clear();
     if (use_single) {
         const TileEntry* k = k1;
         Pass ps;
@@ -88,7 +90,8 @@ int ragged_long_class(gfftx_ragged_s* r, long long L, const std::vector<long lon
         if (!multi) return fail(GFFTX_ERR_UNSUPPORTED, "ragged batch: no kernel for length %lld", L);
         long long ns = 1;
         for (size_t i = 0; i < lens.size(); ++i) {
-            const TileEntry* k = find_tile(r->prec, lens[i], true, i > 0);
+            const int tnb = i < t_tiles.size() ? t_tiles[i].first : 0, twg = i < t_tiles.size() ? t_tiles[i].second : 0;
+            const TileEntry* k = find_tile(r->prec, lens[i], true, i > 0, tnb, twg);
             if (!k) return fail(GFFTX_ERR_UNSUPPORTED, "ragged batch: no pass kernel L=%d", lens[i]);
             Pass ps;
             ps.k = k;

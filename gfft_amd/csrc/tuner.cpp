@@ -148,6 +148,72 @@ int time_plan(gfftx_plan_s* p, void* a, void* b, hipStream_t s, int reps, float*
     return GFFTX_SUCCESS;
 }
 
+// Tuning candidates from pass splits: each split with its default tiles, then with every
+// other compiled tile shape of its first pass (T->E) and of each length it uses after the
+// first pass (T->T), one position at a time ([lens..., 0, nb, wg per pass], 0 = default).
+static std::vector<std::vector<int>> split_variants(int prec, const std::vector<std::vector<int>>& splits)
+{
+    std::vector<std::vector<int>> cands;
+    for (const std::vector<int>& sp : splits) {
+        cands.push_back(sp);
+        std::vector<int> seen;
+        for (size_t i = 0; i < sp.size(); ++i) {
+            const int L = sp[i];
+            const bool later = i > 0;
+            if (later && std::find(seen.begin(), seen.end(), L) != seen.end()) continue;
+            if (later) seen.push_back(L);
+            const TileEntry* def = find_tile(prec, L, true, later);
+            for (int t = 0; t < kNumTiles; ++t) {
+                const TileEntry& e = kTiles[t];
+                if (e.prec != prec || e.L != L || !e.loadT || e.storeT != later || &e == def) continue;
+                std::vector<int> c = sp;
+                c.push_back(0);
+                for (size_t j = 0; j < sp.size(); ++j) {
+                    const bool alt = later ? (j > 0 && sp[j] == L) : j == 0;
+                    c.push_back(alt ? e.nb : 0);
+                    c.push_back(alt ? e.wg : 0);
+                }
+                cands.push_back(c);
+            }
+        }
+    }
+    return cands;
+}
+
+// The candidates vary one pass position at a time around the default tiles.  For the winning
+// split, the fastest variant timed at each position combined (ms[i] < 0: candidate i failed);
+// empty when that is no new candidate.
+static std::vector<int> best_combination(const std::vector<int>& chosen, const std::vector<std::vector<int>>& cands,
+                                         const std::vector<float>& ms)
+{
+    std::vector<int> wl;
+    std::vector<std::pair<int, int>> wt;
+    split_choice(chosen, wl, wt);
+    std::vector<std::pair<int, int>> bt(wl.size(), {0, 0});
+    std::vector<float> bms(wl.size(), 1e30f);
+    for (size_t i = 0; i < cands.size(); ++i) {
+        if (ms[i] < 0) continue;
+        std::vector<int> cl;
+        std::vector<std::pair<int, int>> ct;
+        split_choice(cands[i], cl, ct);
+        if (cl != wl) continue;
+        for (size_t j = 0; j < ct.size() && j < wl.size(); ++j)
+            if (ct[j].first && ms[i] < bms[j]) {
+                bms[j] = ms[i];
+                bt[j] = ct[j];
+            }
+    }
+    std::vector<int> combo = wl;
+    combo.push_back(0);
+    for (const auto& t : bt) {
+        combo.push_back(t.first);
+        combo.push_back(t.second);
+    }
+    const bool varied = std::any_of(bt.begin(), bt.end(), [](const std::pair<int, int>& t) { return t.first != 0; });
+    if (!varied || combo == chosen || std::find(cands.begin(), cands.end(), combo) != cands.end()) return {};
+    return combo;
+}
+
 int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttype, int place, int device,
                 uint32_t flags)
 {
@@ -182,32 +248,7 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
         std::vector<std::vector<int>> splits;
         if (!single_pass)
             splits = candidate_passes(p->prec, n, (size_t)env_int("GFFTX_TUNE_CANDIDATES", 5), batch, device_cus(device));
-        // each split with its default tiles, then with every other compiled tile shape of
-        // its first pass (T->E) and of each length it uses after the first pass (T->T)
-        std::vector<std::vector<int>> cands;
-        for (const std::vector<int>& sp : splits) {
-            cands.push_back(sp);
-            std::vector<int> seen;
-            for (size_t i = 0; i < sp.size(); ++i) {
-                const int L = sp[i];
-                const bool later = i > 0;
-                if (later && std::find(seen.begin(), seen.end(), L) != seen.end()) continue;
-                if (later) seen.push_back(L);
-                const TileEntry* def = find_tile(p->prec, L, true, later);
-                for (int t = 0; t < kNumTiles; ++t) {
-                    const TileEntry& e = kTiles[t];
-                    if (e.prec != p->prec || e.L != L || !e.loadT || e.storeT != later || &e == def) continue;
-                    std::vector<int> c = sp;
-                    c.push_back(0);
-                    for (size_t j = 0; j < sp.size(); ++j) {
-                        const bool alt = later ? (j > 0 && sp[j] == L) : j == 0;
-                        c.push_back(alt ? e.nb : 0);
-                        c.push_back(alt ? e.wg : 0);
-                    }
-                    cands.push_back(c);
-                }
-            }
-        }
+        std::vector<std::vector<int>> cands = split_variants(p->prec, splits);
         if (single_pass)
             for (int t = 0; t < kNumTiles; ++t) {
                 const TileEntry& e = kTiles[t];
@@ -271,34 +312,9 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
             return ms;
         };
         for (size_t i = 0; i < cands.size(); ++i) cms[i] = try_cand(cands[i]);
-        // The candidates vary one pass position at a time around the default tiles.  Combine,
-        // for the winning split, the fastest variant seen at each position, and time that too.
         if (!single_pass && !chosen.empty()) {
-            std::vector<int> wl;
-            std::vector<std::pair<int, int>> wt;
-            split_choice(chosen, wl, wt);
-            std::vector<std::pair<int, int>> bt(wl.size(), {0, 0});
-            std::vector<float> bms(wl.size(), 1e30f);
-            for (size_t i = 0; i < cands.size(); ++i) {
-                if (cms[i] < 0) continue;
-                std::vector<int> cl;
-                std::vector<std::pair<int, int>> ct;
-                split_choice(cands[i], cl, ct);
-                if (cl != wl) continue;
-                for (size_t j = 0; j < ct.size() && j < wl.size(); ++j)
-                    if (ct[j].first && cms[i] < bms[j]) {
-                        bms[j] = cms[i];
-                        bt[j] = ct[j];
-                    }
-            }
-            std::vector<int> combo = wl;
-            combo.push_back(0);
-            for (const auto& t : bt) {
-                combo.push_back(t.first);
-                combo.push_back(t.second);
-            }
-            const bool varied = std::any_of(bt.begin(), bt.end(), [](const std::pair<int, int>& t) { return t.first != 0; });
-            if (varied && combo != chosen && std::find(cands.begin(), cands.end(), combo) == cands.end()) try_cand(combo);
+            const std::vector<int> combo = best_combination(chosen, cands, cms);
+            if (!combo.empty()) try_cand(combo);
         }
         timings += "]";
         (void)hipStreamSynchronize(s);
@@ -327,7 +343,7 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
 // like plans, under the class's own key; a single-pass choice is stored as [L, 1, 0, nb, wg]
 // and returned with *single set and the tile's shape in *nb, *wg.
 bool tune_length_class(int device, int prec, long long L, long long count, bool inverse, bool* single,
-                       std::vector<int>* lens, int* nb, int* wg)
+                       std::vector<int>* lens, std::vector<std::pair<int, int>>* tiles)
 {
     if (env_int("GFFTX_AUTOTUNE", 1) == 0) return false;
     const bool has_single = L <= (1 << 30) && find_tile(prec, (int)L, false, false) != nullptr;
@@ -348,7 +364,7 @@ bool tune_length_class(int device, int prec, long long L, long long count, bool 
             const TileEntry& e = kTiles[t];
             if (e.prec == prec && e.L == L && !e.loadT && !e.storeT) cands.push_back({(int)L, 1, 0, e.nb, e.wg});
         }
-        for (const std::vector<int>& sp : splits) cands.push_back(sp);
+        for (const std::vector<int>& sp : split_variants(prec, splits)) cands.push_back(sp);
         DeviceGuard g(device);
         // at most 1 GiB of data, like plan tuning
         const uint64_t tcount = std::min<uint64_t>((uint64_t)count,
@@ -365,21 +381,28 @@ bool tune_length_class(int device, int prec, long long L, long long count, bool 
         const int cvt = prec == PREC_F64 ? GFFTX_COMPLEX_DOUBLE : GFFTX_COMPLEX_FLOAT;
         float best = 1e30f;
         bool found = false;
-        for (const std::vector<int>& c : cands) {
+        auto try_cand = [&](const std::vector<int>& c) -> float {
             gfftx_plan cp = nullptr;
             // in place, as the classes of a ragged batch run
             if (create_plan_impl(&cp, (uint64_t)L, tcount, cvt, inverse ? GFFTX_IDFT : GFFTX_DFT, GFFTX_IN_PLACE,
                                  device, GFFTX_FLAG_NO_AUTOTUNE, c, "") != GFFTX_SUCCESS)
-                continue;
+                return -1.f;
             float ms = 0;
             const int trc = time_plan(cp, a, a, s, 5, &ms);
             gfftx_plan_destroy(cp);
-            if (trc) continue;
+            if (trc) return -1.f;
             if (ms < best) {
                 best = ms;
                 chosen = c;
                 found = true;
             }
+            return ms;
+        };
+        std::vector<float> cms(cands.size(), -1.f);
+        for (size_t i = 0; i < cands.size(); ++i) cms[i] = try_cand(cands[i]);
+        if (found && !(chosen.size() >= 2 && chosen[1] == 1 && chosen[0] == (int)L)) {
+            const std::vector<int> combo = best_combination(chosen, cands, cms);
+            if (!combo.empty()) try_cand(combo);
         }
         (void)hipStreamSynchronize(s);
         (void)hipStreamDestroy(s);
@@ -390,11 +413,8 @@ bool tune_length_class(int device, int prec, long long L, long long count, bool 
         g_tune[key] = chosen;
         tune_store();
     }
-    std::vector<std::pair<int, int>> tiles;
-    split_choice(chosen, *lens, tiles);
+    split_choice(chosen, *lens, *tiles);
     *single = lens->size() == 2 && (*lens)[1] == 1;
-    *nb = *single && !tiles.empty() ? tiles[0].first : 0;
-    *wg = *single && !tiles.empty() ? tiles[0].second : 0;
     if (*single) lens->clear();
     return true;
 }

@@ -336,11 +336,15 @@ def test_single_pass_tiles_tuned_and_cached(torch_cuda, tmp_path, monkeypatch, d
 
 
 @pytest.mark.parametrize("dtype,n,passes,tnb,twg", [
-    (np.complex128, 2 ** 16, "256,256", 16, 512), (np.complex128, 2 ** 24, "256,256,256", 16, 512)])
+    (np.complex128, 2 ** 16, "256,256", 16, 512), (np.complex128, 2 ** 24, "256,256,256", 16, 512),
+    (np.complex64, 192 * 192, "192,192", 16, 192), (np.complex64, 192 * 192, "192,192", 32, 384),
+    (np.complex64, 320 * 320, "320,320", 16, 320), (np.complex64, 80 * 80, "80,80", 32, 160),
+    (np.complex64, 80 * 80, "80,80", 48, 240)])
 def test_forced_pass_tiles(torch_cuda, monkeypatch, dtype, n, passes, tnb, twg):
-    """Pass tiles forced through GFFTX_TNB / GFFTX_TWG (here the FP64 256-point tile with
-    radix 8x8x4 stages and 512 threads): forward against the oracle (numpy's FFT at 2^24),
-    and the inverse back to the input."""
+    """Pass tiles forced through GFFTX_TNB / GFFTX_TWG (the FP64 256-point tile with radix
+    8x8x4 stages and 512 threads; the FP32 192-, 320- and 80-point tiles whose rows are whole
+    128-byte lines): forward against the oracle (numpy's FFT above 2^16), and the inverse back
+    to the input."""
     monkeypatch.setenv("GFFTX_PASSES", passes)
     monkeypatch.setenv("GFFTX_TNB", str(tnb))
     monkeypatch.setenv("GFFTX_TWG", str(twg))
