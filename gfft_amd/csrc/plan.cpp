@@ -481,7 +481,8 @@ int exec_graph(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
 
 int exec_dispatch(gfftx_plan_s* p, const void* in, void* out, hipStream_t s)
 {
-    const bool din = is_device_ptr(in), dout = is_device_ptr(out);
+    // one pointer query per distinct array (an in-place call asks once)
+    const bool din = is_device_ptr(in), dout = out == in ? din : is_device_ptr(out);
     if (din && dout && (p->flags & GFFTX_FLAG_GRAPH)) return exec_graph(p, in, out, s);
     // complex elements the caller's arrays hold: the real side of a real transform has n
     // per item, the complex side dist (n, or n+1 with the bins layout)

@@ -255,6 +255,15 @@ def test_graph_cache_rotating_buffers(torch_cuda):
             pg.fft(srcs[i], outs[i])
             torch_cuda.cuda.synchronize()
             assert np.array_equal(host(outs[i]), refs[i]), f"call {k} buffer {i}"
+        # the same rotation issued back to back, nothing waited for between the calls: an
+        # evicted graph may still be running when its entry is reused
+        for o in outs:
+            o.zero_()
+        for i in order:
+            pg.fft(srcs[i], outs[i])
+        torch_cuda.cuda.synchronize()
+        for i in range(nbuf):
+            assert np.array_equal(host(outs[i]), refs[i]), f"back to back, buffer {i}"
 
 
 def test_autotuner_measures_then_caches(torch_cuda, tmp_path, monkeypatch):
