@@ -68,12 +68,17 @@ template <int L_, int NB_, int SCALE_ = MFMA_SCALE_STAGE, int WG_ = 256> struct 
     }
 };
 
+// Twiddle entries the kernel keeps in LDS.  Stage 1 (Ns = 16) multiplies row m of column j
+// by W_256^{(j mod 16) m}; stage 2 (Ns = 256) by W_4096^{j m} = W_256^{jb m} W_4096^{c m} for
+// j = 16 jb + c (c, m < 16), formed in FP64 and rounded once.
+constexpr int MFMA_TW = 512;
+
 struct MfmaArgs {
     const float2* in;
     float2* out;
     long long ntrans;
-    const float2* tw;       // stage twiddles, rows of 16 entries (same layout as the tile kernel);
-                            // times 2^-4 for MFMA_SCALE_STAGE kernels
+    const float2* tw;       // MFMA_TW entries: W_256^m (m < 256), times 2^-4 for MFMA_SCALE_STAGE
+                            // kernels, then W_4096^m (m < 256); the kernel keeps them in LDS
     const _Float16* amat;   // MFMA_A_HALFS halves
     int conj_in;
     int conj_out;
@@ -117,8 +122,8 @@ __device__ __forceinline__ int stage_shift(float m)
 }
 
 template <class C, int S>
-__device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const half8_t (&A)[2][2], long long t0,
-                                           int (&sh)[C::BPW])
+__device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const float2* twl, const half8_t (&A)[2][2],
+                                           long long t0, int (&sh)[C::BPW])
 {
     constexpr int NSS = C::ns(S);
     constexpr bool FIRST = (S == 0), LAST = (S == C::NS - 1);
@@ -202,16 +207,27 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
             for (int i = 0; i < 4; ++i) v[i] = rd[i * RS];
         }
         if constexpr (!FIRST) {
-            const int k = j % NSS;
-            const float2* row = a.tw + C::twoff(S) + k * 16 + 4 * g;
-            if constexpr (C::SCALE == MFMA_SCALE_STAGE) {
-                // the row carries the stage's 2^-4, so W^0 is not 1 here
+            static_assert(S <= 2, "lengths up to 4096");
+            float2 w[4];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) v[i] = cmul(v[i], row[i]);
+            for (int i = 0; i < 4; ++i) {
+                const int m = 4 * g + i;
+                if constexpr (S == 1) {
+                    w[i] = twl[c * m];
+                } else {
+                    const float2 p = twl[((j >> 4) * m) & 255], q = twl[256 + c * m];
+                    const double2 d = dmul(double2{p.x, p.y}, double2{q.x, q.y});
+                    w[i] = float2{(float)d.x, (float)d.y};
+                }
+            }
+            if constexpr (C::SCALE == MFMA_SCALE_STAGE) {
+                // the entries carry the stage's 2^-4, so W^0 is not 1 here
+#pragma unroll
+                for (int i = 0; i < 4; ++i) v[i] = cmul(v[i], w[i]);
             } else {
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    if (g != 0 || i != 0) v[i] = cmul(v[i], row[i]);
+                    if (g != 0 || i != 0) v[i] = cmul(v[i], w[i]);
             }
         }
         float sc = 1.0f, inv = 1.0f;
@@ -305,18 +321,21 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
 }
 
 template <class C, int S>
-__device__ __forceinline__ void mfma_loop(const MfmaArgs& a, float2* lds, const half8_t (&A)[2][2], long long t0,
-                                          int (&sh)[C::BPW])
+__device__ __forceinline__ void mfma_loop(const MfmaArgs& a, float2* lds, const float2* twl, const half8_t (&A)[2][2],
+                                          long long t0, int (&sh)[C::BPW])
 {
     if constexpr (S < C::NS) {
-        mfma_stage<C, S>(a, lds, A, t0, sh);
-        mfma_loop<C, S + 1>(a, lds, A, t0, sh);
+        mfma_stage<C, S>(a, lds, twl, A, t0, sh);
+        mfma_loop<C, S + 1>(a, lds, twl, A, t0, sh);
     }
 }
 
 template <class C> __global__ __launch_bounds__(C::WG) void mfma_fft_kernel(MfmaArgs a)
 {
     __shared__ float2 lds[C::LDS_ELEMS];
+    __shared__ float2 twl[MFMA_TW];
+    // read by the stages after the first, behind its barriers
+    for (int i = threadIdx.x; i < MFMA_TW; i += C::WG) twl[i] = a.tw[i];
     const int lane = threadIdx.x & 63;
     half8_t A[2][2];
 #pragma unroll
@@ -326,7 +345,7 @@ template <class C> __global__ __launch_bounds__(C::WG) void mfma_fft_kernel(Mfma
 #pragma unroll
             for (int i = 0; i < 8; ++i) A[mt][p][i] = a.amat[((mt * 2 + p) * 64 + lane) * 8 + i];
     int sh[C::BPW] = {};  // log2 of each block's first-stage scale (per-stage scaling)
-    mfma_loop<C, 0>(a, lds, A, (long long)blockIdx.x * C::NB, sh);
+    mfma_loop<C, 0>(a, lds, twl, A, (long long)blockIdx.x * C::NB, sh);
 }
 
 }  // namespace gfftx

@@ -73,11 +73,9 @@ int build_c2c_passes(gfftx_plan_s* p)
         ps.mfma = me;
         ps.items_in_ntrans = true;
         ps.ntrans = (long long)p->batch;
-        const int rad[4] = {16, 16, 16, 16};
         // per-stage scaling: each stage after the first takes its input times 2^-4 through
         // the twiddles (exact: a power of two)
-        int rc = upload_stage_table_r(p->allocs, PREC_F32, me->L == 256 ? 2 : 3, rad, &ps.stage_tw,
-                                      me->scale == MFMA_SCALE_STAGE ? 0.0625 : 1.0);
+        int rc = upload_mfma_twiddles(p->allocs, me->scale == MFMA_SCALE_STAGE ? 0.0625 : 1.0, &ps.stage_tw);
         if (rc) return rc;
         rc = upload_mfma_matrix(p->allocs, &ps.amat);
         if (rc) return rc;

@@ -94,6 +94,7 @@ int upload_stage_table_r(std::vector<void*>& owned, int prec, int nrad, const in
                          double scale = 1.0);
 int upload_root_tables(std::vector<void*>& owned, long long M, double2** lo_out, double2** hi_out, int* shift);
 int upload_mfma_matrix(std::vector<void*>& owned, void** out);
+int upload_mfma_twiddles(std::vector<void*>& owned, double scale, void** out);
 
 // Instantiated hipGraphs keyed by the (in, out) device pointers they were captured with.  A
 // few entries let a caller rotate over several buffers without re-capturing; the least

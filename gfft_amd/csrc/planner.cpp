@@ -281,9 +281,25 @@ int upload_stage_table_r(std::vector<void*>& owned, int prec, int nrad, const in
         std::vector<float> t;
         pack_stage_table(nrad, radices, t);
         for (float& v : t) v = (float)(v * scale);
-        int rc = dev_alloc_list(owned, out, t.size() * sizeof(float));
+        // then, 16-byte aligned, the FP64 roots W_{Ns R}^k (k < Ns) of every stage s >= 1,
+        // from which the FP32 tiles form their twiddles (TileCfg::TW_POW)
+        while (t.size() % 4) t.push_back(0.f);
+        std::vector<double> roots;
+        long long ns = radices[0];
+        for (int st = 1; st < nrad; ++st) {
+            for (long long kk = 0; kk < ns; ++kk) {
+                long double c, si;
+                root_ld(kk, ns * radices[st], &c, &si);
+                roots.push_back((double)c);
+                roots.push_back((double)si);
+            }
+            ns *= radices[st];
+        }
+        const size_t rows = t.size() * sizeof(float), bytes = rows + roots.size() * sizeof(double);
+        int rc = dev_alloc_list(owned, out, bytes);
         if (rc) return rc;
-        HIP_TRY(hipMemcpy(*out, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(*out, t.data(), rows, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy((char*)*out + rows, roots.data(), roots.size() * sizeof(double), hipMemcpyHostToDevice));
     }
     return GFFTX_SUCCESS;
 }
@@ -318,6 +334,29 @@ int upload_root_tables(std::vector<void*>& owned, long long M, double2** lo_out,
     *lo_out = (double2*)dlo;
     *hi_out = (double2*)dhi;
     *shift = b;
+    return GFFTX_SUCCESS;
+}
+
+// Twiddles of the MFMA kernel (MfmaArgs::tw): W_256^m times `scale` (the per-stage 2^-4 or 1),
+// then W_4096^m, m < 256.
+int upload_mfma_twiddles(std::vector<void*>& owned, double scale, void** out)
+{
+    std::vector<float> t;
+    for (int m = 0; m < 256; ++m) {
+        long double c, si;
+        root_ld(m, 256, &c, &si);
+        t.push_back((float)((double)c * scale));
+        t.push_back((float)((double)si * scale));
+    }
+    for (int m = 0; m < 256; ++m) {
+        long double c, si;
+        root_ld(m, 4096, &c, &si);
+        t.push_back((float)c);
+        t.push_back((float)si);
+    }
+    int rc = dev_alloc_list(owned, out, t.size() * sizeof(float));
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(*out, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
     return GFFTX_SUCCESS;
 }
 

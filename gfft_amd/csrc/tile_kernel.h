@@ -93,6 +93,18 @@ struct TileCfg {
         return o;
     }
     static constexpr int twsize() { return twoff(NS); }
+    // FP32 tiles form a stage's twiddles W^{k r}, r = 1..R-1, as FP64 powers of one FP64 root
+    // W_{Ns R}^k (then rounded to FP32): one 16-byte load per butterfly instead of R-1 table
+    // entries.  The roots follow the rows, 16-byte aligned, stage s at twbase(s) = the sum of
+    // Ns over the earlier stages.
+    static constexpr bool TW_POW = sizeof(T_) == 4;
+    static constexpr int twbase(int s)
+    {
+        int o = 0;
+        for (int i = 1; i < s; ++i) o += ns(i);
+        return o;
+    }
+    static constexpr int TW_ROOT_OFF = (twoff(NS) * (int)sizeof(V2<T_>) + 15) / 16 * 16;  // bytes
     // LDS: one padded element every 16, odd distance between sequences (bank spread).
     static constexpr int LP = L + L / 16;
     static constexpr int TS = (LP % 2 == 0) ? LP + 1 : LP;
@@ -293,9 +305,39 @@ template <class C, bool IDLE, int S> __device__ __forceinline__ void tile_stage(
 #pragma unroll
             for (int r = 0; r < R; ++r) v[b][r] = x.lds[C::lidx(x.t, j + r * LR)];
             const int k = j % NSS;
-            const V* row = x.tw + C::twoff(S) + k * R;
+            if constexpr (C::TW_POW) {
+                const double2 w1 = reinterpret_cast<const double2*>(reinterpret_cast<const char*>(x.tw) +
+                                                                    C::TW_ROOT_OFF)[C::twbase(S) + k];
+                if constexpr (R >= 8 && R % 4 == 0) {
+                    // W^{k (4a + c)} = (W^{4k})^a W^{k c}: short dependency chains (the
+                    // stage's latency), at most four FP64 roundings per twiddle
+                    const double2 w2 = dmul(w1, w1), w3 = dmul(w2, w1), w4 = dmul(w2, w2);
+                    v[b][1] = cmul(v[b][1], to_v<V>(w1));
+                    v[b][2] = cmul(v[b][2], to_v<V>(w2));
+                    v[b][3] = cmul(v[b][3], to_v<V>(w3));
+                    double2 wa = w4;
 #pragma unroll
-            for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], row[r]);
+                    for (int a = 1; a < R / 4; ++a) {
+                        if (a > 1) wa = dmul(wa, w4);
+                        v[b][4 * a] = cmul(v[b][4 * a], to_v<V>(wa));
+                        v[b][4 * a + 1] = cmul(v[b][4 * a + 1], to_v<V>(dmul(wa, w1)));
+                        v[b][4 * a + 2] = cmul(v[b][4 * a + 2], to_v<V>(dmul(wa, w2)));
+                        v[b][4 * a + 3] = cmul(v[b][4 * a + 3], to_v<V>(dmul(wa, w3)));
+                    }
+                } else {
+                    double2 w = w1;
+                    v[b][1] = cmul(v[b][1], to_v<V>(w));
+#pragma unroll
+                    for (int r = 2; r < R; ++r) {
+                        w = dmul(w, w1);
+                        v[b][r] = cmul(v[b][r], to_v<V>(w));
+                    }
+                }
+            } else {
+                const V* row = x.tw + C::twoff(S) + k * R;
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], row[r]);
+            }
         }
         Dft<R>::run(v[b]);
     }
