@@ -1,6 +1,8 @@
 // planner.cpp -- twiddle roots, compiled-tile lookup, the pass cost model and pass splits,
 // the chirp-z convolution length, and the device tables plans upload.
 #include "plan_internal.h"
+
+#include <cstring>
 #include "roots.h"
 
 namespace gfftx {
@@ -270,37 +272,41 @@ int upload_stage_table_r(std::vector<void*>& owned, int prec, int nrad, const in
 {
     *out = nullptr;
     if (nrad <= 1) return GFFTX_SUCCESS;
+    // the rows in the working precision, then, 16-byte aligned, the FP64 roots W_{Ns R}^k
+    // (k < Ns) of every stage s >= 1, from which the tiles form their twiddles
+    // (TileCfg::TW_POW)
+    std::vector<double> roots;
+    long long ns = radices[0];
+    for (int st = 1; st < nrad; ++st) {
+        for (long long kk = 0; kk < ns; ++kk) {
+            long double c, si;
+            root_ld(kk, ns * radices[st], &c, &si);
+            roots.push_back((double)c);
+            roots.push_back((double)si);
+        }
+        ns *= radices[st];
+    }
+    std::vector<unsigned char> bytes;
     if (prec == PREC_F64) {
         std::vector<double> t;
         pack_stage_table(nrad, radices, t);
         for (double& v : t) v *= scale;
-        int rc = dev_alloc_list(owned, out, t.size() * sizeof(double));
-        if (rc) return rc;
-        HIP_TRY(hipMemcpy(*out, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+        bytes.resize(t.size() * sizeof(double));
+        memcpy(bytes.data(), t.data(), bytes.size());
     } else {
         std::vector<float> t;
         pack_stage_table(nrad, radices, t);
         for (float& v : t) v = (float)(v * scale);
-        // then, 16-byte aligned, the FP64 roots W_{Ns R}^k (k < Ns) of every stage s >= 1,
-        // from which the FP32 tiles form their twiddles (TileCfg::TW_POW)
         while (t.size() % 4) t.push_back(0.f);
-        std::vector<double> roots;
-        long long ns = radices[0];
-        for (int st = 1; st < nrad; ++st) {
-            for (long long kk = 0; kk < ns; ++kk) {
-                long double c, si;
-                root_ld(kk, ns * radices[st], &c, &si);
-                roots.push_back((double)c);
-                roots.push_back((double)si);
-            }
-            ns *= radices[st];
-        }
-        const size_t rows = t.size() * sizeof(float), bytes = rows + roots.size() * sizeof(double);
-        int rc = dev_alloc_list(owned, out, bytes);
-        if (rc) return rc;
-        HIP_TRY(hipMemcpy(*out, t.data(), rows, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy((char*)*out + rows, roots.data(), roots.size() * sizeof(double), hipMemcpyHostToDevice));
+        bytes.resize(t.size() * sizeof(float));
+        memcpy(bytes.data(), t.data(), bytes.size());
     }
+    const size_t rows = bytes.size();
+    bytes.resize(rows + roots.size() * sizeof(double));
+    memcpy(bytes.data() + rows, roots.data(), roots.size() * sizeof(double));
+    int rc = dev_alloc_list(owned, out, bytes.size());
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(*out, bytes.data(), bytes.size(), hipMemcpyHostToDevice));
     return GFFTX_SUCCESS;
 }
 

@@ -94,9 +94,9 @@ struct TileCfg {
     }
     static constexpr int twsize() { return twoff(NS); }
     // FP32 tiles form a stage's twiddles W^{k r}, r = 1..R-1, as FP64 powers of one FP64 root
-    // W_{Ns R}^k (then rounded to FP32): one 16-byte load per butterfly instead of R-1 table
+    // W_{Ns R}^k, rounded to FP32 once: one 16-byte load per butterfly instead of R-1 table
     // entries.  The roots follow the rows, 16-byte aligned, stage s at twbase(s) = the sum of
-    // Ns over the earlier stages.
+    // Ns over the earlier stages.  FP64 tiles read the rows (the powers measured no faster).
     static constexpr bool TW_POW = sizeof(T_) == 4;
     static constexpr int twbase(int s)
     {
