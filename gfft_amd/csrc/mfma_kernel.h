@@ -70,7 +70,7 @@ template <int L_, int NB_, int SCALE_ = MFMA_SCALE_STAGE, int WG_ = 256> struct 
 
 // Twiddle entries the kernel keeps in LDS.  Stage 1 (Ns = 16) multiplies row m of column j
 // by W_256^{(j mod 16) m}; stage 2 (Ns = 256) by W_4096^{j m} = W_256^{jb m} W_4096^{c m} for
-// j = 16 jb + c (c, m < 16), formed in FP64 and rounded once.
+// j = 16 jb + c (c, m < 16), the product of two table entries in FP32.
 constexpr int MFMA_TW = 512;
 
 struct MfmaArgs {
@@ -215,9 +215,7 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
                 if constexpr (S == 1) {
                     w[i] = twl[c * m];
                 } else {
-                    const float2 p = twl[((j >> 4) * m) & 255], q = twl[256 + c * m];
-                    const double2 d = dmul(double2{p.x, p.y}, double2{q.x, q.y});
-                    w[i] = float2{(float)d.x, (float)d.y};
+                    w[i] = cmul(twl[((j >> 4) * m) & 255], twl[256 + c * m]);
                 }
             }
             if constexpr (C::SCALE == MFMA_SCALE_STAGE) {
