@@ -188,6 +188,8 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
         }
     }
 
+    half8_t bh[C::BPW], bl[C::BPW];  // B operands of every block (hi and lo parts)
+    float invs[C::BPW];              // per-column scaling: each block's unscale
 #pragma unroll
     for (int b = 0; b < C::BPW; ++b) {
         const int blk = wave * C::BPW + b;             // block index over all NB sequences
@@ -248,7 +250,7 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
             // component instead of a conversion of hi and a subtraction
             asm volatile("" : "+v"(sc));
         }
-        half8_t bh, bl;
+        invs[b] = inv;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             // hi = FP16(x s); lo = FP16(x s - hi): x s - hi is formed by one mixed-precision
@@ -257,26 +259,37 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
             const float2_t xs = (C::SCALE == MFMA_SCALE_STAGE && !FIRST) ? float2_t{v[i].x, v[i].y}
                                                                          : float2_t{v[i].x * sc, v[i].y * sc};
             const half2_t h = __builtin_convertvector(xs, half2_t);
-            bh[2 * i] = h.x;
-            bh[2 * i + 1] = h.y;
-            bl[2 * i] = (_Float16)__builtin_fmaf(v[i].x, sc, -(float)h.x);
-            bl[2 * i + 1] = (_Float16)__builtin_fmaf(v[i].y, sc, -(float)h.y);
+            bh[b][2 * i] = h.x;
+            bh[b][2 * i + 1] = h.y;
+            bl[b][2 * i] = (_Float16)__builtin_fmaf(v[i].x, sc, -(float)h.x);
+            bl[b][2 * i + 1] = (_Float16)__builtin_fmaf(v[i].y, sc, -(float)h.y);
         }
-        float4_t acc[2];
+    }
+    // the products of all blocks, each round over every block before the next, so that no
+    // product waits for the one before it: A B ~= Al Bh + Ah Bl + Ah Bh (smallest terms
+    // first) for the Re (mt = 0) and Im (mt = 1) rows
+    float4_t acc[C::BPW][2];
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt) {
-            float4_t z = {0.f, 0.f, 0.f, 0.f};
-            z = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[mt][1], bh, z, 0, 0, 0);
-            z = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[mt][0], bl, z, 0, 0, 0);
-            z = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[mt][0], bh, z, 0, 0, 0);
-            acc[mt] = z;
-        }
+    for (int b = 0; b < C::BPW; ++b)
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+            acc[b][mt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[mt][1], bh[b], float4_t{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+#pragma unroll
+    for (int b = 0; b < C::BPW; ++b)
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) acc[b][mt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[mt][0], bl[b], acc[b][mt], 0, 0, 0);
+#pragma unroll
+    for (int b = 0; b < C::BPW; ++b)
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) acc[b][mt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[mt][0], bh[b], acc[b][mt], 0, 0, 0);
+#pragma unroll
+    for (int b = 0; b < C::BPW; ++b) {
         if constexpr (C::SCALE == MFMA_SCALE_COLUMN) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) y[b][i] = float2{acc[0][i] * inv, acc[1][i] * inv};
+            for (int i = 0; i < 4; ++i) y[b][i] = float2{acc[b][0][i] * invs[b], acc[b][1][i] * invs[b]};
         } else {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) y[b][i] = float2{acc[0][i], acc[1][i]};
+            for (int i = 0; i < 4; ++i) y[b][i] = float2{acc[b][0][i], acc[b][1][i]};
         }
     }
 
