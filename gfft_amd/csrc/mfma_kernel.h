@@ -130,7 +130,6 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int c = lane & 15, g = lane >> 4;
     float2 y[C::BPW][4];
-    const unsigned cmask_in = a.conj_in ? 0x80000000u : 0u;
     // first stage of per-stage scaling: all inputs are loaded first, then the largest
     // component of each transform sets its scale
     float2 x0[FIRST ? C::BPW : 1][4];
@@ -146,7 +145,7 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
             const float2* src = a.in + (tg < a.ntrans ? tg : a.ntrans - 1) * C::L;
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-                x0[b][i] = flip_im(nt_load(&src[(unsigned)(j + (4 * g + i) * C::COLS)]), cmask_in);
+                x0[b][i] = nt_load(&src[(unsigned)(j + (4 * g + i) * C::COLS)]);
         }
         if constexpr (C::SCALE == MFMA_SCALE_STAGE) {
             // blocks per sequence: COLS/16; a wave holds BPW blocks, so a sequence spans
@@ -251,18 +250,20 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
             asm volatile("" : "+v"(sc));
         }
         invs[b] = inv;
+        // the inverse's conjugation of the input rides on the first stage's scale of Im
+        const float scy = (FIRST && a.conj_in) ? -sc : sc;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             // hi = FP16(x s); lo = FP16(x s - hi): x s - hi is formed by one mixed-precision
             // FMA from x, s and the FP16 hi (the product by a power of two is exact; s = 1
             // after the first stage of per-stage scaling)
             const float2_t xs = (C::SCALE == MFMA_SCALE_STAGE && !FIRST) ? float2_t{v[i].x, v[i].y}
-                                                                         : float2_t{v[i].x * sc, v[i].y * sc};
+                                                                         : float2_t{v[i].x * sc, v[i].y * scy};
             const half2_t h = __builtin_convertvector(xs, half2_t);
             bh[b][2 * i] = h.x;
             bh[b][2 * i + 1] = h.y;
             bl[b][2 * i] = (_Float16)__builtin_fmaf(v[i].x, sc, -(float)h.x);
-            bl[b][2 * i + 1] = (_Float16)__builtin_fmaf(v[i].y, sc, -(float)h.y);
+            bl[b][2 * i + 1] = (_Float16)__builtin_fmaf(v[i].y, scy, -(float)h.y);
         }
     }
     // the products of all blocks, each round over every block before the next, so that no
@@ -295,7 +296,6 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
 
     if constexpr (!FIRST) __syncthreads();
 
-    const unsigned cmask_out = a.conj_out ? 0x80000000u : 0u;
 #pragma unroll
     for (int b = 0; b < C::BPW; ++b) {
         const int blk = wave * C::BPW + b;
@@ -308,14 +308,11 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
             if (tg < a.ntrans) {
                 float2* dst = a.out + tg * C::L;
                 // per-stage scaling: undo s0 and the 2^-4 of each later stage
+                // (the inverse's conjugation of the output rides on the scale of Im)
                 const float sc = C::SCALE == MFMA_SCALE_STAGE ? ldexpf(a.scale, 4 * (C::NS - 1) - sh[b]) : a.scale;
+                const float scy = a.conj_out ? -sc : sc;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    float2 u = y[b][i];
-                    u.x *= sc;
-                    u.y *= sc;
-                    dst[(unsigned)(base + (4 * g + i) * NSS)] = flip_im(u, cmask_out);
-                }
+                for (int i = 0; i < 4; ++i) dst[(unsigned)(base + (4 * g + i) * NSS)] = float2{y[b][i].x * sc, y[b][i].y * scy};
             }
         } else {
             // lidx(t, base + m NSS), m = 4g + i: for NSS = 1, base = 16 j and m < 16; for NSS
