@@ -263,7 +263,11 @@ int ragged_build(gfftx_ragged_s* r, const uint64_t* lengths, uint64_t count)
         memset(&rc, 0, sizeof(rc));
         rc.L = L;
         rc.count = (long long)members[c].size();
-        const int single = r->prec == PREC_F32 ? ragged_single_cfg(L) : -1;
+        // lengths up to GFFTX_RAGGED_PERSISTENT_MAX (default 4096, the largest persistent
+        // tile) run in the persistent kernel; 0 sends every length to the class launches
+        const int single = (r->prec == PREC_F32 && L <= env_int("GFFTX_RAGGED_PERSISTENT_MAX", 4096))
+                               ? ragged_single_cfg(L)
+                               : -1;
         if (single < 0 && (mode == 0 || r->prec != PREC_F32)) {
             int e = ragged_long_class(r, L, members[c], env_int("GFFTX_RAGGED_MIN_SINGLE", 0), &nscratch);
             if (e) return e;
