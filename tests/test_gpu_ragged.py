@@ -57,6 +57,25 @@ def test_all_lengths_once(torch_cuda, monkeypatch, mode):
     p.close()
 
 
+def test_all_lengths_as_class_launches(torch_cuda, monkeypatch):
+    """GFFTX_RAGGED_PERSISTENT_MAX=0: no persistent kernel, every length (16 to 65536) a
+    batched class launch on the side streams; forward and inverse against the oracle."""
+    monkeypatch.setenv("GFFTX_RAGGED_PERSISTENT_MAX", "0")
+    lengths = [16, 48, 80, 192, 256] + C4_LENGTHS + [4096, 320, 1536]
+    x = inputs.complex_normal(sum(lengths), 78).astype(np.complex64)
+    p = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT)
+    assert '"items":[0,0]' in p.describe().replace(" ", "")
+    d = dev(torch_cuda, x)
+    p.fft(d)
+    y = host(d)
+    check_all(y, x, lengths)
+    p.close()
+    q = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.IDFT)
+    q.fft(d)
+    check_all(host(d), y.astype(np.complex128), lengths, inverse=True)
+    q.close()
+
+
 @pytest.mark.parametrize("mode", MODES)
 def test_inverse_round_trip(torch_cuda, monkeypatch, mode):
     monkeypatch.setenv("GFFTX_RAGGED_MODE", mode)
