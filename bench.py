@@ -11,7 +11,10 @@ Also reported on the same line:
                 gives the longest launch's share of those bytes (2 N b_c / passes) over its
                 average duration from HIP events attached to its dispatches
                 (GFFTX_FLAG_PROFILE), and `per_pass_*` the bytes one pass moves (2 N b_c)
-                over the same duration;
+                over the same duration; `dominant_kernel_all_launches` the same for the
+                kernel with the most time per step over all its launches (forward and
+                inverse passes that run the same kernel together), as a profiler's
+                per-kernel statistics group them;
   errors        every workload's forward result against numpy's complex128 FFT of the same
                 (rounded) input, computed outside the timed regions: relative L2, largest
                 absolute component error and GFFT's max|err|/max|y| over re and im
