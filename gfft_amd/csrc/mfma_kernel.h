@@ -122,7 +122,7 @@ __device__ __forceinline__ int stage_shift(float m)
 }
 
 template <class C, int S>
-__device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const float2* twl, const half8_t (&A)[2][2],
+__device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, float2* twl, const half8_t (&A)[2][2],
                                            long long t0, int (&sh)[C::BPW])
 {
     constexpr int NSS = C::ns(S);
@@ -147,6 +147,13 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
             for (int i = 0; i < 4; ++i)
                 x0[b][i] = nt_load(&src[(unsigned)(j + (4 * g + i) * C::COLS)]);
         }
+        // the twiddle table is loaded behind the sequences, so that a workgroup issues its
+        // data loads first, and stored to the LDS at the end of this block; the later stages
+        // read it behind the barrier that ends this stage
+        static_assert(MFMA_TW % C::WG == 0, "whole rounds of the table copy");
+        float2 tv[MFMA_TW / C::WG];
+#pragma unroll
+        for (int k = 0; k < MFMA_TW / C::WG; ++k) tv[k] = a.tw[threadIdx.x + k * C::WG];
         if constexpr (C::SCALE == MFMA_SCALE_STAGE) {
             // blocks per sequence: COLS/16; a wave holds BPW blocks, so a sequence spans
             // (COLS/16)/BPW waves (one: the wave's blocks are whole sequences)
@@ -185,6 +192,8 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
                 }
             }
         }
+#pragma unroll
+        for (int k = 0; k < MFMA_TW / C::WG; ++k) twl[threadIdx.x + k * C::WG] = tv[k];
     }
 
     half8_t bh[C::BPW], bl[C::BPW];  // B operands of every block (hi and lo parts)
@@ -329,7 +338,7 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, const
 }
 
 template <class C, int S>
-__device__ __forceinline__ void mfma_loop(const MfmaArgs& a, float2* lds, const float2* twl, const half8_t (&A)[2][2],
+__device__ __forceinline__ void mfma_loop(const MfmaArgs& a, float2* lds, float2* twl, const half8_t (&A)[2][2],
                                           long long t0, int (&sh)[C::BPW])
 {
     if constexpr (S < C::NS) {
@@ -341,9 +350,7 @@ __device__ __forceinline__ void mfma_loop(const MfmaArgs& a, float2* lds, const 
 template <class C> __global__ __launch_bounds__(C::WG) void mfma_fft_kernel(MfmaArgs a)
 {
     __shared__ float2 lds[C::LDS_ELEMS];
-    __shared__ float2 twl[MFMA_TW];
-    // read by the stages after the first, behind its barriers
-    for (int i = threadIdx.x; i < MFMA_TW; i += C::WG) twl[i] = a.tw[i];
+    __shared__ float2 twl[MFMA_TW];  // filled by the first stage, read by the later ones
     const int lane = threadIdx.x & 63;
     half8_t A[2][2];
 #pragma unroll
