@@ -169,6 +169,9 @@ TileEntry kTiles[] = {
     // 1024 points with eight per thread in the first two stages (512 threads): the
     // autotuner weighs it against the radix-16 tile for batched FP64 transforms
     GFFTX_TILE(PREC_F64, double, 1024, 4, 512, false, false, 8, 8, 16),
+    // one sequence per 64-thread workgroup: 17 KB of LDS each, nine sequences per CU where
+    // the four-sequence tile fits eight
+    GFFTX_TILE(PREC_F64, double, 1024, 1, 64, false, false, 16, 16, 4),
     GFFTX_TILE(PREC_F64, double, 256, 16, 512, true, false, 8, 8, 4),
     GFFTX_TILE(PREC_F64, double, 256, 16, 512, true, true, 8, 8, 4),
     GFFTX_TILE(PREC_F64, double, 256, 8, 128, true, false, 16, 16),
