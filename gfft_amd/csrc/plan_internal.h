@@ -331,6 +331,10 @@ int create_plan(gfftx_plan* out, uint64_t n, uint64_t batch, int vtype, int ttyp
                 uint32_t flags);
 bool tune_length_class(int device, int prec, long long L, long long count, bool inverse, bool* single,
                        std::vector<int>* lens, std::vector<std::pair<int, int>>* tiles);
+// cache entries of other choices (the ragged batch's persistent-kernel limit)
+std::string ragged_tune_key(int device, int prec, const uint64_t* lengths, uint64_t count);
+bool tune_get(const std::string& key, std::vector<int>* v);
+void tune_put(const std::string& key, const std::vector<int>& v);
 
 }  // namespace detail
 }  // namespace gfftx

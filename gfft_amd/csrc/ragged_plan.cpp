@@ -220,7 +220,10 @@ void measure_class_costs(gfftx_ragged_s* r, long long points, std::vector<double
     if (ok) cost = t;
 }
 
-int ragged_build(gfftx_ragged_s* r, const uint64_t* lengths, uint64_t count)
+// Builds the plan of a ragged batch.  FP32 lengths up to `pmax` (at most 4096, the largest
+// persistent tile) run in the persistent kernel; longer ones, or all with pmax = 0, run as
+// batched class launches on the side streams.
+int ragged_build(gfftx_ragged_s* r, const uint64_t* lengths, uint64_t count, long long pmax)
 {
     // classes by length, in order of first appearance
     std::vector<long long> lens;
@@ -263,11 +266,7 @@ int ragged_build(gfftx_ragged_s* r, const uint64_t* lengths, uint64_t count)
         memset(&rc, 0, sizeof(rc));
         rc.L = L;
         rc.count = (long long)members[c].size();
-        // lengths up to GFFTX_RAGGED_PERSISTENT_MAX (default 4096, the largest persistent
-        // tile) run in the persistent kernel; 0 sends every length to the class launches
-        const int single = (r->prec == PREC_F32 && L <= env_int("GFFTX_RAGGED_PERSISTENT_MAX", 4096))
-                               ? ragged_single_cfg(L)
-                               : -1;
+        const int single = (r->prec == PREC_F32 && L <= pmax) ? ragged_single_cfg(L) : -1;
         if (single < 0 && (mode == 0 || r->prec != PREC_F32)) {
             int e = ragged_long_class(r, L, members[c], env_int("GFFTX_RAGGED_MIN_SINGLE", 0), &nscratch);
             if (e) return e;
@@ -510,6 +509,91 @@ int ragged_launch(gfftx_ragged_s* r, void* data, hipStream_t s)
     return GFFTX_SUCCESS;
 }
 
+// Time (ms) of one whole call of a ragged plan, direct launches issued back to back on `s`
+// over `buf`: after two warm-up calls, three groups of three calls between events, the
+// smallest group mean.  Negative when a launch fails.
+double time_ragged(gfftx_ragged_s* r, void* buf, hipStream_t s)
+{
+    RaggedArgs a = r->args;
+    a.data = (float2*)buf;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        (void)hipGetLastError();
+        return -1.0;
+    }
+    double best = -1.0;
+    bool ok = true;
+    for (int w = 0; ok && w < 2; ++w) ok = ragged_launch_all(r, a, s) == GFFTX_SUCCESS;
+    for (int g = 0; ok && g < 3; ++g) {
+        ok = hipEventRecord(e0, s) == hipSuccess;
+        for (int c = 0; ok && c < 3; ++c) ok = ragged_launch_all(r, a, s) == GFFTX_SUCCESS;
+        float ms = 0;
+        ok = ok && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+             hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+        if (ok && (best < 0 || ms / 3 < best)) best = ms / 3;
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipGetLastError();
+    return ok ? best : -1.0;
+}
+
+// Chooses between the persistent kernel for the lengths up to 4096 (plan `r`, built with
+// that limit) and class launches for every length: builds the second form, times one call
+// of each twice in turn on a zeroed array of the batch's size, keeps the faster in `*r` and
+// records the limit under `key`.  Leaves `*r` as it is when the second form or the array
+// cannot be had.  Returns the note for describe().
+std::string choose_persistent(gfftx_ragged_s** r, const uint64_t* lengths, uint64_t count, const std::string& key)
+{
+    gfftx_ragged_s* p = *r;
+    gfftx_ragged_s* q = new (std::nothrow) gfftx_ragged_s();
+    if (!q) return "";
+    q->device = p->device;
+    q->prec = p->prec;
+    q->inverse = p->inverse;
+    q->flags = p->flags;
+    if (ragged_build(q, lengths, count, 0) != GFFTX_SUCCESS) {
+        gfftx_ragged_destroy(q);
+        return "";
+    }
+    const size_t bytes = (size_t)p->total * elem_bytes(p->prec);
+    void* buf = nullptr;
+    hipStream_t s = nullptr;
+    double tp = -1, tq = -1;
+    if (hipMalloc(&buf, bytes) == hipSuccess && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess &&
+        hipMemsetAsync(buf, 0, bytes, s) == hipSuccess) {
+        for (int rep = 0; rep < 2; ++rep) {
+            const double a = time_ragged(p, buf, s), b = time_ragged(q, buf, s);
+            if (a < 0 || b < 0) {
+                tp = tq = -1;
+                break;
+            }
+            tp = (rep == 0 || a < tp) ? a : tp;
+            tq = (rep == 0 || b < tq) ? b : tq;
+        }
+    }
+    if (s) (void)hipStreamDestroy(s);
+    if (buf) (void)hipFree(buf);
+    (void)hipGetLastError();
+    if (tp < 0 || tq < 0) {
+        gfftx_ragged_destroy(q);
+        return "";
+    }
+    const bool classes = tq < tp;
+    tune_put(key, {classes ? 0 : 4096});
+    if (classes) {
+        gfftx_ragged_destroy(p);
+        *r = q;
+    } else {
+        gfftx_ragged_destroy(q);
+    }
+    char note[128];
+    snprintf(note, sizeof(note), "{\"persistent_ms\":%.4f,\"class_launches_ms\":%.4f}", tp, tq);
+    return note;
+}
+
 int ragged_exec_any(gfftx_ragged_s* plan, void* data, hipStream_t s)
 {
     if (is_device_ptr(data)) return ragged_launch(plan, data, s);
@@ -559,10 +643,41 @@ int gfftx_ragged_create(gfftx_ragged* plan, const uint64_t* lengths, uint64_t co
     r->prec = (vtype == GFFTX_DOUBLE || vtype == GFFTX_COMPLEX_DOUBLE) ? PREC_F64 : PREC_F32;
     r->inverse = ttype == GFFTX_IDFT;
     r->flags = flags;
-    int rc = ragged_build(r, lengths, count);
+    // FP32 lengths up to `pmax` run in the persistent kernel.  GFFTX_RAGGED_PERSISTENT_MAX
+    // fixes the limit; otherwise the autotuner times the batch both ways (persistent kernel
+    // for the lengths up to 4096, or class launches for every length) and caches the choice.
+    // Without autotuning, and in the other grouping modes, the limit is 4096.
+    long long pmax = 4096;
+    std::string key, note = "\"default\"";
+    bool measure = false;
+    if (getenv("GFFTX_RAGGED_PERSISTENT_MAX")) {
+        pmax = env_int("GFFTX_RAGGED_PERSISTENT_MAX", 4096);
+        note = "\"environment\"";
+    } else if (r->prec == PREC_F32 && !(flags & GFFTX_FLAG_NO_AUTOTUNE) && env_int("GFFTX_AUTOTUNE", 1) != 0 &&
+               env_int("GFFTX_RAGGED_MODE", 0) == 0) {
+        key = ragged_tune_key(device, r->prec, lengths, count);
+        std::vector<int> v;
+        if (tune_get(key, &v) && v.size() == 1) {
+            pmax = v[0];
+            note = "\"cache\"";
+        } else {
+            measure = true;
+        }
+    }
+    int rc = ragged_build(r, lengths, count, pmax);
     if (rc != GFFTX_SUCCESS) {
         gfftx_ragged_destroy(r);  // leaves the error message alone
         return rc;
+    }
+    if (measure && r->args.nitems0 + r->args.nitems1 > 0) {
+        const std::string m = choose_persistent(&r, lengths, count, key);
+        if (!m.empty()) note = m;
+    }
+    // describe(): the limit in force and how it was set
+    if (!r->desc.empty() && r->desc.back() == '}') {
+        r->desc.pop_back();
+        r->desc += ",\"persistent_max\":" + std::to_string(r->args.nitems0 + r->args.nitems1 > 0 ? pmax : 0) +
+                   ",\"persistent_choice\":" + note + "}";
     }
     *plan = r;
     return GFFTX_SUCCESS;

@@ -50,7 +50,9 @@ void read_entries(const std::string& path, std::map<std::string, std::vector<int
         std::vector<int> lens;
         std::vector<std::pair<int, int>> tiles;
         split_choice(v, lens, tiles);
-        if (lens.size() >= 2) out[key] = v;
+        // plan entries hold two pass lengths or more; a ragged batch's entry one limit
+        const bool ragged = key.find("|ragged|") != std::string::npos;
+        if (lens.size() >= 2 || (ragged && v.size() == 1)) out[key] = v;
     }
     fclose(f);
 }
@@ -98,6 +100,52 @@ void tune_store()
     fclose(f);
     (void)rename(tmp.c_str(), path.c_str());
     if (lk >= 0) close(lk);  // releases the lock
+}
+
+std::string tune_key(int device, int prec, uint64_t n, uint64_t batch);
+
+// Key of a ragged batch's choice: the device, the precision, the point and transform counts
+// and an FNV-1a hash of the (length, count) classes in increasing length.
+std::string ragged_tune_key(int device, int prec, const uint64_t* lengths, uint64_t count)
+{
+    std::map<uint64_t, uint64_t> classes;
+    uint64_t points = 0;
+    for (uint64_t i = 0; i < count; ++i) {
+        ++classes[lengths[i]];
+        points += lengths[i];
+    }
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&h](uint64_t v) {
+        for (int b = 0; b < 8; ++b) {
+            h ^= (v >> (8 * b)) & 0xff;
+            h *= 1099511628211ull;
+        }
+    };
+    for (const auto& kv : classes) {
+        mix(kv.first);
+        mix(kv.second);
+    }
+    char hex[24];
+    snprintf(hex, sizeof(hex), "%016llx", (unsigned long long)h);
+    return tune_key(device, prec, points, count) + "|ragged|h=" + hex;
+}
+
+bool tune_get(const std::string& key, std::vector<int>* v)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_load();
+    auto it = g_tune.find(key);
+    if (it == g_tune.end()) return false;
+    *v = it->second;
+    return true;
+}
+
+void tune_put(const std::string& key, const std::vector<int>& v)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_load();
+    g_tune[key] = v;
+    tune_store();
 }
 
 std::string tune_key(int device, int prec, uint64_t n, uint64_t batch)

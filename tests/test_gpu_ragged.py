@@ -76,6 +76,63 @@ def test_all_lengths_as_class_launches(torch_cuda, monkeypatch):
     q.close()
 
 
+def test_persistent_kernel_forced(torch_cuda, monkeypatch):
+    """GFFTX_RAGGED_PERSISTENT_MAX=4096: the lengths up to 4096 run in the persistent kernel
+    whatever the autotuner would choose; forward against the oracle."""
+    import json
+
+    monkeypatch.setenv("GFFTX_RAGGED_PERSISTENT_MAX", "4096")
+    lengths = [16, 48, 80, 192, 256] + C4_LENGTHS + [4096, 320, 1536]
+    x = inputs.complex_normal(sum(lengths), 79).astype(np.complex64)
+    p = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT)
+    desc = json.loads(p.describe())
+    assert desc["items"][0] > 0 and desc["persistent_max"] == 4096 and desc["persistent_choice"] == "environment"
+    d = dev(torch_cuda, x)
+    p.fft(d)
+    check_all(host(d), x, lengths)
+    p.close()
+
+
+def test_persistent_choice_measured_and_cached(torch_cuda, tmp_path, monkeypatch):
+    """Without GFFTX_RAGGED_PERSISTENT_MAX the autotuner times the batch with the persistent
+    kernel and with class launches for every length, keeps the faster and caches the limit
+    under the batch's classes; the next plan of the same classes reads it.  Both plans match
+    the oracle."""
+    import json
+
+    monkeypatch.delenv("GFFTX_RAGGED_PERSISTENT_MAX", raising=False)
+    cache = tmp_path / "tune.json"
+    monkeypatch.setenv("GFFTX_TUNE_CACHE", str(cache))
+    lengths = [int(v) for v in inputs.ragged_lengths(600, 31)]
+    x = inputs.complex_normal(sum(lengths), 80).astype(np.complex64)
+    p = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT)
+    desc = json.loads(p.describe())
+    choice = desc["persistent_choice"]
+    print("ragged persistent choice", desc["persistent_max"], choice)
+    assert isinstance(choice, dict) and choice["persistent_ms"] > 0 and choice["class_launches_ms"] > 0
+    faster_classes = choice["class_launches_ms"] < choice["persistent_ms"]
+    assert desc["persistent_max"] == (0 if faster_classes else 4096)
+    assert (desc["items"] == [0, 0]) == faster_classes
+    entries = [ln for ln in cache.read_text().splitlines() if "|ragged|h=" in ln]
+    assert len(entries) == 1 and entries[0].rstrip(",").endswith(f"[{desc['persistent_max']}]"), entries
+    d = dev(torch_cuda, x)
+    p.fft(d)
+    check_all(host(d), x, lengths)
+    p.close()
+    q = api.RaggedPlan(lengths[::-1], api.COMPLEX_FLOAT, api.DFT)  # same classes, other order
+    d2 = json.loads(q.describe())
+    assert d2["persistent_choice"] == "cache" and d2["persistent_max"] == desc["persistent_max"]
+    xr = np.concatenate([x[sum(lengths[:i]):sum(lengths[:i + 1])] for i in range(len(lengths) - 1, -1, -1)])
+    e = dev(torch_cuda, xr)
+    q.fft(e)
+    check_all(host(e), xr, lengths[::-1])
+    q.close()
+    r = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT, flags=api.FLAG_NO_AUTOTUNE)
+    d3 = json.loads(r.describe())
+    r.close()
+    assert d3["persistent_choice"] == "default" and d3["persistent_max"] == 4096
+
+
 @pytest.mark.parametrize("mode", MODES)
 def test_inverse_round_trip(torch_cuda, monkeypatch, mode):
     monkeypatch.setenv("GFFTX_RAGGED_MODE", mode)
