@@ -366,6 +366,7 @@ def other_workloads(torch, api):
     xg_h = inputs.complex_normal(tot, 4096).astype(np.complex64)
     xg = torch.from_numpy(xg_h).cuda()
     rp = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT, flags=api.FLAG_GRAPH)
+    d4 = json.loads(rp.describe())
     rp.fft(xg)
     e4 = ragged_errors(xg.cpu().numpy(), xg_h, lengths)
     xg.copy_(torch.from_numpy(xg_h))
@@ -379,6 +380,8 @@ def other_workloads(torch, api):
     res["c4_ragged_4096_f32"] = {"us": round(us, 1), "launch": "hipGraph replay", "direct_us": round(us_direct, 1),
                                  "GFLOPs": round(flops / us / 1e3, 1),
                                  "GBps_alg": round(2 * tot * 8 / us / 1e3, 1), "points": tot,
+                                 "persistent_max": d4.get("persistent_max"),
+                                 "persistent_choice": d4.get("persistent_choice"),
                                  "errors_vs_numpy_fp64": e4}
     del xg, xg_h
     # C5: 16384 x 4096 FP32 plain vs split-precision MFMA; accuracy against the FP64 engine
