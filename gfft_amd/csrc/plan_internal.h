@@ -268,8 +268,9 @@ struct gfftx_plan_s {
 };
 
 // ---------------------------------------------------------------------------------------
-// Ragged batches: one persistent launch over a list of work items (ragged.hip).
-// Side streams for the long length classes: with the caller's stream they fill the four
+// Ragged batches: one persistent launch over a list of work items (ragged.hip) for the
+// lengths up to the plan's limit, batched class launches for the others.
+// Side streams for the class launches: with the caller's stream they fill the four
 // hardware queues a process has by default.
 constexpr int kRaggedSides = 3;
 
@@ -282,10 +283,11 @@ struct gfftx_ragged_s {
     std::vector<void*> allocs;
     RaggedArgs args{};
     int grid = 0, grid1 = 0;  // workgroups of the phase-0 and phase-1 launches
-    // Grouped mode (default): each length class above 4096 points runs as its own batched
-    // launch sequence (a single-pass tile through the sequence-offset table, or multi-pass
-    // tiles through a batch-offset table) on a side stream, while the persistent kernel
-    // takes the shorter lengths on the caller's stream.
+    // Grouped mode (default): each length class above the persistent limit (4096 points, or
+    // 0 where the autotuner found class launches faster) runs as its own batched launch
+    // sequence (a single-pass tile through the sequence-offset table, or multi-pass tiles
+    // through a batch-offset table) on a side stream, while the persistent kernel takes the
+    // shorter lengths on the caller's stream.
     struct LongClass {
         long long L = 0;
         long long count = 0;

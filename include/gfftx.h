@@ -159,14 +159,17 @@ int gfftx_plan_profile(gfftx_plan plan, double* total_ms, uint64_t* count, const
  * in one persistent launch; every other length class (all of them in FP64) runs as batched
  * launches on side streams that overlap it (a class with a prime factor above 5 as a
  * chirp-z plan over its transforms); one call (or one hipGraph with GFFTX_FLAG_GRAPH)
- * covers all.
+ * covers all.  With autotuning on, plan creation also times the batch with every length as
+ * a class launch and keeps that form where it is faster (the choice is cached per set of
+ * length classes; GFFTX_RAGGED_PERSISTENT_MAX fixes it).
  * No reference counterpart: the nearest is one GFFT object per length from one
  * GenerateTransform<NList,...> factory (src/gfftgen.h:189-233) called in a loop. */
 int gfftx_ragged_create(gfftx_ragged* plan, const uint64_t* lengths, uint64_t count, int vtype,
                         int ttype, int device, uint32_t flags);
 int gfftx_ragged_exec(gfftx_ragged plan, void* data, void* stream);
 int gfftx_ragged_destroy(gfftx_ragged plan);
-/* JSON description of a ragged plan (classes, items, grid).  No reference counterpart. */
+/* JSON description of a ragged plan (classes, items, grid, the persistent-kernel limit
+ * and how it was chosen).  No reference counterpart. */
 int gfftx_ragged_describe(gfftx_ragged plan, char* buf, size_t len);
 /* Summed duration and count of ragged launches since the last query (GFFTX_FLAG_PROFILE).
  * No reference counterpart. */
