@@ -198,6 +198,13 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, float
 
     half8_t bh[C::BPW], bl[C::BPW];  // B operands of every block (hi and lo parts)
     float invs[C::BPW];              // per-column scaling: each block's unscale
+    // the twiddle factors that do not depend on the block, read once: W_256^{c m} (stage 1),
+    // W_4096^{c m} (stage 2)
+    float2 wc[4];
+    if constexpr (!FIRST) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) wc[i] = twl[(S == 1 ? 0 : 256) + c * (4 * g + i)];
+    }
 #pragma unroll
     for (int b = 0; b < C::BPW; ++b) {
         const int blk = wave * C::BPW + b;             // block index over all NB sequences
@@ -223,9 +230,9 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, float
             for (int i = 0; i < 4; ++i) {
                 const int m = 4 * g + i;
                 if constexpr (S == 1) {
-                    w[i] = twl[c * m];
+                    w[i] = wc[i];
                 } else {
-                    w[i] = cmul(twl[((j >> 4) * m) & 255], twl[256 + c * m]);
+                    w[i] = cmul(twl[((j >> 4) * m) & 255], wc[i]);
                 }
             }
             if constexpr (C::SCALE == MFMA_SCALE_STAGE) {
