@@ -83,6 +83,7 @@ struct MfmaArgs {
     int conj_in;
     int conj_out;
     float scale;
+    int nt_out;             // 1: the output is stored non-temporally (put())
 };
 
 __device__ __forceinline__ float col_max4(float m)
@@ -327,8 +328,12 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, float
                 // (the inverse's conjugation of the output rides on the scale of Im)
                 const float sc = C::SCALE == MFMA_SCALE_STAGE ? ldexpf(a.scale, 4 * (C::NS - 1) - sh[b]) : a.scale;
                 const float scy = a.conj_out ? -sc : sc;
+                with_nt(a.nt_out != 0, [&](auto ntc) {
 #pragma unroll
-                for (int i = 0; i < 4; ++i) dst[(unsigned)(base + (4 * g + i) * NSS)] = float2{y[b][i].x * sc, y[b][i].y * scy};
+                    for (int i = 0; i < 4; ++i)
+                        put<decltype(ntc)::value>(&dst[(unsigned)(base + (4 * g + i) * NSS)],
+                                                  float2{y[b][i].x * sc, y[b][i].y * scy});
+                });
             }
         } else {
             // lidx(t, base + m NSS), m = 4g + i: for NSS = 1, base = 16 j and m < 16; for NSS

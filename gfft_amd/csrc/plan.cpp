@@ -32,6 +32,12 @@ int fail(int code, const char* fmt, ...)
     return code;
 }
 
+bool nt_output(size_t bytes)
+{
+    const long long mib = env_int("GFFTX_NT_STORE_MIB", 256);
+    return mib >= 0 && bytes > (size_t)mib << 20;
+}
+
 int dev_alloc(gfftx_plan_s* p, void** out, size_t bytes)
 {
     if (bytes == 0) bytes = 16;
@@ -288,6 +294,7 @@ int run_passes(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, Range 
             ma.conj_in = p->inverse ? 1 : 0;
             ma.conj_out = p->inverse ? 1 : 0;
             ma.scale = p->inverse ? (float)(1.0 / (double)p->n) : 1.0f;
+            ma.nt_out = (r.nt && i == m - 1) ? 1 : 0;
             dim3 grid((unsigned)((ma.ntrans + ps.mfma->nb - 1) / ps.mfma->nb));
             std::pair<hipEvent_t, hipEvent_t> ev;
             const bool on = prof_begin(prof, slot0 + i, ps.mfma->name, s, &ev, true);
@@ -314,6 +321,7 @@ int run_passes(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, Range 
         a.conj_out = (p->inverse && i == m - 1) ? 1 : 0;
         a.scale = (p->inverse && i == m - 1) ? 1.0 / (double)p->n : 1.0;
         a.xcd = ps.xcd;
+        a.nt_out = (r.nt && i == m - 1) ? 1 : 0;
         const long long nb = ps.k->nb;
         unsigned ntiles = (unsigned)((a.ntrans + nb - 1) / nb);
         if (ps.real) {
@@ -364,6 +372,7 @@ int blue_launch(gfftx_plan_s* p, int which, const BluesteinArgs& a, size_t slot,
 int run_bluestein(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, Range r, const long long* boff)
 {
     if (!r.nb) r.nb = p->batch;
+    r.nt = false;  // the inner transforms write the work array, which the next kernel reads
     const size_t eb = elem_bytes(p->prec);
     BluesteinArgs a;
     memset(&a, 0, sizeof(a));
@@ -425,10 +434,18 @@ int run_split(gfftx_plan_s* p, void* data, hipStream_t s, Range r)
 // `r` selects batch items (all by default); in and out are the base pointers of the batch.
 int exec_device(gfftx_plan_s* p, const void* in, void* out, hipStream_t s, Range r)
 {
-    if (!r.nb) r.nb = p->batch;
+    if (!r.nb) {
+        // a call over the whole batch on device arrays: nothing in the call reads its output
+        // back, so a large one is stored non-temporally (the host path's staged chunks are
+        // copied out right after, and stay cached)
+        r.nb = p->batch;
+        r.nt = nt_output((size_t)p->dist * p->batch * elem_bytes(p->prec));
+    }
     if (!p->real || p->split_fused) return run_c2c(p, in, out, s, r);
     if (!p->inverse) {
-        int rc = run_c2c(p, in, out, s, r);
+        Range rt = r;
+        rt.nt = false;  // the split kernel reads the transform's output
+        int rc = run_c2c(p, in, out, s, rt);
         if (rc) return rc;
         return run_split(p, out, s, r);
     }

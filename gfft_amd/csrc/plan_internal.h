@@ -71,7 +71,12 @@ struct Pass {
 // Batch items [b0, b0 + nb) of a plan; nb == 0 means every item.
 struct Range {
     unsigned long long b0 = 0, nb = 0;
+    bool nt = false;  // the last pass stores the call's output non-temporally
 };
+
+// Whether an output of `bytes` is stored non-temporally: larger than the 256 MB Infinity
+// Cache (GFFTX_NT_STORE_MIB, default 256; negative: never).
+bool nt_output(size_t bytes);
 
 inline size_t elem_bytes(int prec) { return prec == PREC_F64 ? 16 : 8; }
 

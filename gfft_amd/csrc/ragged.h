@@ -61,6 +61,8 @@ struct RaggedArgs {
     int nitems0, nitems1;
     int inverse;
     int phase;                    // set per launch
+    int nt_out;                   // 1: final outputs stored non-temporally (a batch larger
+                                  // than the Infinity Cache)
 };
 
 struct RaggedCfgInfo {

@@ -120,6 +120,7 @@ __global__ __launch_bounds__(RAGGED_WG) void ragged_kernel(RaggedArgs ra)
         a.seq_off = nullptr;
         a.boff = nullptr;
         a.scale = 1.0;
+        a.nt_out = 0;
         if (item.kind == RAGGED_SINGLE) {
             // tile of same-length transforms, in place through the offset table
             a.in = ra.data;
@@ -132,6 +133,7 @@ __global__ __launch_bounds__(RAGGED_WG) void ragged_kernel(RaggedArgs ra)
             a.conj_in = ra.inverse;
             a.conj_out = ra.inverse;
             a.scale = ra.inverse ? 1.0 / (double)cls.L : 1.0;
+            a.nt_out = ra.nt_out;
         } else {
             // first pass of a two-pass transform: A-point FFTs of x[j + P r] -> z[q P + j]
             const long long A = cls.A, P = cls.L / A;
@@ -164,6 +166,7 @@ __global__ __launch_bounds__(RAGGED_WG) void ragged_kernel(RaggedArgs ra)
                 a.conj_in = 0;
                 a.conj_out = ra.inverse;
                 a.scale = ra.inverse ? 1.0 / (double)cls.L : 1.0;
+                a.nt_out = ra.nt_out;
             }
             if (item.kind == RAGGED_FUSED) {
                 for (int t = 0; t < item.nt1; ++t) {

@@ -137,6 +137,7 @@ int ragged_run_long(gfftx_ragged_s* r, const gfftx_ragged_s::LongClass& lc, void
         a.conj_in = (r->inverse && i == 0) ? 1 : 0;
         a.conj_out = (r->inverse && i == m - 1) ? 1 : 0;
         a.scale = (r->inverse && i == m - 1) ? 1.0 / (double)lc.L : 1.0;
+        a.nt_out = (i + 1 == m) ? r->args.nt_out : 0;
         dim3 grid;
         if (ps.items_in_ntrans) {
             a.ntrans = lc.count;
@@ -243,6 +244,9 @@ int ragged_build(gfftx_ragged_s* r, const uint64_t* lengths, uint64_t count, lon
         members[c].push_back(offsets[i]);
     }
     r->total = (unsigned long long)off;
+    // the transforms' final outputs are stored non-temporally when the batch is larger than
+    // the Infinity Cache (nothing in the call reads them back)
+    r->args.nt_out = nt_output((size_t)off * elem_bytes(r->prec)) ? 1 : 0;
     std::vector<RaggedClass> classes(lens.size());
     struct HostItem {
         RaggedItem it;

@@ -21,6 +21,8 @@
 // memory (its == 1 on load, Ns > 1 on store); "E" (element-contiguous) means each sequence
 // is contiguous.  Lanes are mapped so that consecutive lanes touch consecutive addresses.
 #pragma once
+#include <type_traits>
+
 #include "fft_common.h"
 
 namespace gfftx {
@@ -50,6 +52,7 @@ struct TileArgs {
     int sp_shift;
     int split_bins;            // 1: N+1 bins, 0: NR packing
     long long nfull;           // N, the complex length of the transform
+    int nt_out;                // 1: the output is stored non-temporally (see put())
 };
 
 __device__ __forceinline__ long long batch_offset(const TileArgs& a, unsigned by)
@@ -202,6 +205,29 @@ template <class V> __device__ __forceinline__ V nt_load(const V* p)
     return V{r.x, r.y};
 }
 
+// Store of one complex element of a launch's output: non-temporal when `nt` (the host sets it
+// for the last pass of a call whose output is larger than the 256 MB Infinity Cache and is
+// not read back by the call: caching those lines would only evict what the next loads need).
+template <bool NT, class V> __device__ __forceinline__ void put(V* p, V v)
+{
+    if constexpr (NT) {
+        using E = decltype(V{}.x);
+        typedef E vec __attribute__((ext_vector_type(2)));
+        __builtin_nontemporal_store(vec{v.x, v.y}, reinterpret_cast<vec*>(p));
+    } else {
+        *p = v;
+    }
+}
+
+// Runs f with the store kind as a compile-time constant, so that each store loop is emitted
+// once per kind and issues its stores back to back (a run-time test per store would put a
+// branch between them).
+template <class F> __device__ __forceinline__ void with_nt(bool nt, F&& f)
+{
+    if (nt) f(std::true_type{});
+    else f(std::false_type{});
+}
+
 // Conjugation as a sign-bit flip of the imaginary part: mask is 0 or the sign bit.
 __device__ __forceinline__ float2 flip_im(float2 u, unsigned mask)
 {
@@ -349,25 +375,29 @@ template <class C, bool IDLE, int S> __device__ __forceinline__ void tile_stage(
         if (x.valid && (!IDLE || x.active)) {
             const TileArgs& a = x.a;
             const unsigned mask = a.conj_out ? 0x80000000u : 0u;
-            if (a.scale == 1.0) {
+            const bool nt = a.nt_out != 0;
+            with_nt(nt, [&](auto ntc) {
+                constexpr bool NT = decltype(ntc)::value;
+                if (a.scale == 1.0) {
 #pragma unroll
-                for (int b = 0; b < B; ++b) {
-                    const int j = x.jt + b * C::TPT;
-                    if (GUARD && j >= LR) continue;
+                    for (int b = 0; b < B; ++b) {
+                        const int j = x.jt + b * C::TPT;
+                        if (GUARD && j >= LR) continue;
 #pragma unroll
-                    for (int r = 0; r < R; ++r) x.out[(unsigned)(j + r * LR) * x.ons] = flip_im(v[b][r], mask);
+                        for (int r = 0; r < R; ++r) put<NT>(&x.out[(unsigned)(j + r * LR) * x.ons], flip_im(v[b][r], mask));
+                    }
+                } else {
+                    const auto sc = (decltype(v[0][0].x))a.scale;
+#pragma unroll
+                    for (int b = 0; b < B; ++b) {
+                        const int j = x.jt + b * C::TPT;
+                        if (GUARD && j >= LR) continue;
+#pragma unroll
+                        for (int r = 0; r < R; ++r)
+                            put<NT>(&x.out[(unsigned)(j + r * LR) * x.ons], flip_im(cscale(v[b][r], sc), mask));
+                    }
                 }
-            } else {
-                const auto sc = (decltype(v[0][0].x))a.scale;
-#pragma unroll
-                for (int b = 0; b < B; ++b) {
-                    const int j = x.jt + b * C::TPT;
-                    if (GUARD && j >= LR) continue;
-#pragma unroll
-                    for (int r = 0; r < R; ++r)
-                        x.out[(unsigned)(j + r * LR) * x.ons] = flip_im(cscale(v[b][r], sc), mask);
-                }
-            }
+            });
         }
     } else {
 #pragma unroll
@@ -460,23 +490,26 @@ template <class C, bool IDLE> __device__ __forceinline__ void tile_finish(const 
         const unsigned mask = a.conj_out ? 0x80000000u : 0u;
         const auto sc = (decltype(V{}.x))a.scale;
         const bool scale = a.scale != 1.0;
+        const bool nt = a.nt_out != 0;
         const int tid = threadIdx.x;
         V* out = reinterpret_cast<V*>(a.out) + x.bo;
         if constexpr (C::PAIRED) {
             // paired sequences p and S - p (the C2R pre-pass of a first pass): element lin of
             // the tile is element q = lin % L of lane lin / L's sequence, stored by the rule
             if (!IDLE || x.active) {
+                with_nt(nt, [&](auto ntc) {
 #pragma unroll
-                for (int i = 0; i < PER; ++i) {
-                    const int lin = tid + i * C::WG;
-                    const int ts = lin / C::L, q = lin % C::L;
-                    const long long tgs = split_seq<C>(a.ntrans, tile, ts);
-                    if (tgs >= 0) {
-                        V u = x.lds[C::lidx(ts, q)];
-                        if (scale) u = cscale(u, sc);
-                        out[out_base<C>(a, tgs) + (unsigned)q * ns] = flip_im(u, mask);
+                    for (int i = 0; i < PER; ++i) {
+                        const int lin = tid + i * C::WG;
+                        const int ts = lin / C::L, q = lin % C::L;
+                        const long long tgs = split_seq<C>(a.ntrans, tile, ts);
+                        if (tgs >= 0) {
+                            V u = x.lds[C::lidx(ts, q)];
+                            if (scale) u = cscale(u, sc);
+                            put<decltype(ntc)::value>(&out[out_base<C>(a, tgs) + (unsigned)q * ns], flip_im(u, mask));
+                        }
                     }
-                }
+                });
             }
             return;
         }
@@ -496,19 +529,24 @@ template <class C, bool IDLE> __device__ __forceinline__ void tile_finish(const 
                     const int lin = tid + i * C::WG;
                     u[i] = x.lds[C::lidx(lin / C::L, lin % C::L)];
                 }
+                with_nt(nt, [&](auto ntc) {
 #pragma unroll
-                for (int i = 0; i < PER; ++i) o[tid + i * C::WG] = flip_im(scale ? cscale(u[i], sc) : u[i], mask);
+                    for (int i = 0; i < PER; ++i)
+                        put<decltype(ntc)::value>(&o[tid + i * C::WG], flip_im(scale ? cscale(u[i], sc) : u[i], mask));
+                });
             } else if (!IDLE || x.active) {
+                with_nt(nt, [&](auto ntc) {
 #pragma unroll
-                for (int i = 0; i < PER; ++i) {
-                    const int lin = tid + i * C::WG;
-                    const int ts = lin / C::L, q = lin % C::L;
-                    if (ts < nvalid) {
-                        V u = x.lds[C::lidx(ts, q)];
-                        if (scale) u = cscale(u, sc);
-                        o[lin] = flip_im(u, mask);
+                    for (int i = 0; i < PER; ++i) {
+                        const int lin = tid + i * C::WG;
+                        const int ts = lin / C::L, q = lin % C::L;
+                        if (ts < nvalid) {
+                            V u = x.lds[C::lidx(ts, q)];
+                            if (scale) u = cscale(u, sc);
+                            put<decltype(ntc)::value>(&o[lin], flip_im(u, mask));
+                        }
                     }
-                }
+                });
             }
             return;
         }
@@ -519,28 +557,32 @@ template <class C, bool IDLE> __device__ __forceinline__ void tile_finish(const 
             const long long tgs = (long long)tile * C::NB + ts;
             if ((!IDLE || x.active) && tgs < a.ntrans) {
                 V* o = out + out_base<C>(a, tgs);
+                with_nt(nt, [&](auto ntc) {
 #pragma unroll
-                for (int i = 0; i < PER; ++i) {
-                    const int q = (tid + i * C::WG) / C::NB;
-                    V u = x.lds[C::lidx(ts, q)];
-                    if (scale) u = cscale(u, sc);
-                    o[(unsigned)q * ns] = flip_im(u, mask);
-                }
+                    for (int i = 0; i < PER; ++i) {
+                        const int q = (tid + i * C::WG) / C::NB;
+                        V u = x.lds[C::lidx(ts, q)];
+                        if (scale) u = cscale(u, sc);
+                        put<decltype(ntc)::value>(&o[(unsigned)q * ns], flip_im(u, mask));
+                    }
+                });
             }
             return;
         }
         if (!IDLE || x.active) {
+            with_nt(nt, [&](auto ntc) {
 #pragma unroll
-            for (int i = 0; i < PER; ++i) {
-                const int lin = tid + i * C::WG;
-                const int ts = lin / C::L, q = lin % C::L;
-                const long long tgs = (long long)tile * C::NB + ts;
-                if (tgs < a.ntrans) {
-                    V u = x.lds[C::lidx(ts, q)];
-                    if (scale) u = cscale(u, sc);
-                    out[out_base<C>(a, tgs) + (unsigned)q * ns] = flip_im(u, mask);
+                for (int i = 0; i < PER; ++i) {
+                    const int lin = tid + i * C::WG;
+                    const int ts = lin / C::L, q = lin % C::L;
+                    const long long tgs = (long long)tile * C::NB + ts;
+                    if (tgs < a.ntrans) {
+                        V u = x.lds[C::lidx(ts, q)];
+                        if (scale) u = cscale(u, sc);
+                        put<decltype(ntc)::value>(&out[out_base<C>(a, tgs) + (unsigned)q * ns], flip_im(u, mask));
+                    }
                 }
-            }
+            });
         }
     }
 }

@@ -373,6 +373,52 @@ def test_forced_pass_tiles(torch_cuda, monkeypatch, dtype, n, passes, tnb, twg):
         assert rel_l2(host(d), x) <= 2 * TOL[dtype]
 
 
+def test_non_temporal_stores_change_no_result(torch_cuda, monkeypatch):
+    """The last pass stores a large output non-temporally (GFFTX_NT_STORE_MIB, default 256):
+    the store kind must not change a single bit.  Each plan runs with the hint forced on for
+    every size (0) and off (-1): batched single-pass tiles, a multi-pass transform (transposed
+    last-pass stores), the split-precision MFMA kernel, a real forward and inverse pair and a
+    ragged batch."""
+    def run(mib):
+        monkeypatch.setenv("GFFTX_NT_STORE_MIB", str(mib))
+        outs = []
+        x = inputs.complex_normal(1024 * 48, 71).astype(np.complex64)
+        with api.Plan(1024, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE, batch=48) as p:
+            d, o = dev(torch_cuda, x), torch_cuda.empty(1024 * 48, dtype=torch_cuda.complex64, device="cuda")
+            p.fft(d, o)
+            outs.append(host(o))
+        x = inputs.complex_normal(2 ** 20, 72)
+        with api.Plan(2 ** 20, api.COMPLEX_DOUBLE, api.DFT, api.IN_PLACE) as p:
+            d = dev(torch_cuda, x)
+            p.fft(d)
+            outs.append(host(d))
+        x = inputs.complex_normal(4096 * 32, 73).astype(np.complex64)
+        with api.Plan(4096, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE, batch=32, flags=api.FLAG_SPLIT_MFMA) as p:
+            d, o = dev(torch_cuda, x), torch_cuda.empty(4096 * 32, dtype=torch_cuda.complex64, device="cuda")
+            p.fft(d, o)
+            outs.append(host(o))
+        xr = inputs.complex_normal(2 ** 19, 74).astype(np.complex64)  # 2^20 reals, interleaved
+        with api.Plan(2 ** 19, api.FLOAT, api.RDFT, api.IN_PLACE) as pf, \
+                api.Plan(2 ** 19, api.FLOAT, api.IRDFT, api.IN_PLACE) as pi:
+            d = dev(torch_cuda, xr)
+            pf.fft(d)
+            outs.append(host(d))
+            pi.fft(d)
+            outs.append(host(d))
+        lengths = [256, 5120, 65536, 384, 640, 12288, 4096]
+        x = inputs.complex_normal(sum(lengths), 75).astype(np.complex64)
+        rp = api.RaggedPlan(lengths, api.COMPLEX_FLOAT, api.DFT)
+        d = dev(torch_cuda, x)
+        rp.fft(d)
+        outs.append(host(d))
+        rp.close()
+        return outs
+
+    on, off = run(0), run(-1)
+    for k, (a, b) in enumerate(zip(on, off)):
+        assert np.array_equal(a, b), f"case {k}: non-temporal stores changed the result"
+
+
 def test_out_of_place_alias_rejected(torch_cuda):
     with api.Plan(256, api.COMPLEX_FLOAT, api.DFT, api.OUT_OF_PLACE) as p:
         d = torch_cuda.zeros(256, dtype=torch_cuda.complex64, device="cuda")
