@@ -50,6 +50,10 @@ template <int L_, int NB_, int SCALE_ = MFMA_SCALE_STAGE, int WG_ = 256> struct 
     static constexpr int NS = (L == 256) ? 2 : (L == 4096) ? 3 : (L == 65536) ? 4 : 0;
     static_assert(NS > 0, "length must be 16^S");
     static_assert(BLOCKS % WAVES == 0 && BPW >= 1, "the waves share the blocks evenly");
+    // the wave's four blocks are consecutive columns of one sequence (the last stage's Ns is
+    // L/16 = COLS, so it writes element (q, j) to q COLS + j): the first loads and the last
+    // stores then move whole rows of the wave's 64 columns
+    static constexpr bool ROWS = BPW == 4 && COLS / 16 >= BPW;
     static constexpr int LP = L + L / 16;
     static constexpr int TS = (LP % 2 == 0) ? LP + 1 : LP;
     static constexpr int LDS_ELEMS = NB * TS;
@@ -113,6 +117,40 @@ __device__ __forceinline__ float wave_max(float m)
     return __uint_as_float(u);
 }
 
+// 4 x 4 transpose between the lane bits 4-5 (g = lane >> 4) and the index of v: lane (c, g)
+// ends with v[b] = the old v[g] of lane (c, b).  gfx950's v_permlane16_swap / v_permlane32_swap
+// exchange two registers between the lanes whose bit 4 / bit 5 differ, which is one step of
+// the transpose by itself (no selects).
+template <int D> __device__ __forceinline__ void swap_lanes(float2& u, float2& w)
+{
+    int ux = __float_as_int(u.x), wx = __float_as_int(w.x), uy = __float_as_int(u.y), wy = __float_as_int(w.y);
+    if constexpr (D == 16) {
+        const auto rx = __builtin_amdgcn_permlane16_swap(ux, wx, false, false);
+        const auto ry = __builtin_amdgcn_permlane16_swap(uy, wy, false, false);
+        ux = rx[0];
+        wx = rx[1];
+        uy = ry[0];
+        wy = ry[1];
+    } else {
+        const auto rx = __builtin_amdgcn_permlane32_swap(ux, wx, false, false);
+        const auto ry = __builtin_amdgcn_permlane32_swap(uy, wy, false, false);
+        ux = rx[0];
+        wx = rx[1];
+        uy = ry[0];
+        wy = ry[1];
+    }
+    u = float2{__int_as_float(ux), __int_as_float(uy)};
+    w = float2{__int_as_float(wx), __int_as_float(wy)};
+}
+
+__device__ __forceinline__ void transpose_lane_block(float2& v0, float2& v1, float2& v2, float2& v3)
+{
+    swap_lanes<16>(v0, v1);  // lane bit 4 with index bit 0
+    swap_lanes<16>(v2, v3);
+    swap_lanes<32>(v0, v2);  // lane bit 5 with index bit 1
+    swap_lanes<32>(v1, v3);
+}
+
 // s0 = 2^(14 - e) for the largest component m = f 2^e (f in [0.5, 1)) of a transform, the
 // exponent clamped so that s0 and its inverse are normal FP32 numbers; returns log2 s0.
 __device__ __forceinline__ int stage_shift(float m)
@@ -135,18 +173,35 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, float
     // component of each transform sets its scale
     float2 x0[FIRST ? C::BPW : 1][4];
     if constexpr (FIRST) {
-#pragma unroll
-        for (int b = 0; b < C::BPW; ++b) {
-            const int blk = wave * C::BPW + b;
-            const int t = blk / (C::COLS / 16);
-            const int j = (blk % (C::COLS / 16)) * 16 + c;
-            const long long tg = t0 + t;
-            // loads past the last sequence are clamped to it; their results are not stored.
-            // The input is read once, so it is loaded non-temporally (whole sequences).
+        if constexpr (C::ROWS) {
+            // lane (c, g) loads column 16 g + c of the wave's 64 for rows 4 b + i (every load
+            // a contiguous 512-byte row piece); the transpose between g and b then gives it
+            // column 16 b + c of block b for rows 4 g + i, the layout of the products
+            const int blk0 = wave * C::BPW;
+            const long long tg = t0 + blk0 / (C::COLS / 16);
+            const int j0 = (blk0 % (C::COLS / 16)) * 16;
             const float2* src = a.in + (tg < a.ntrans ? tg : a.ntrans - 1) * C::L;
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-                x0[b][i] = nt_load(&src[(unsigned)(j + (4 * g + i) * C::COLS)]);
+            for (int b = 0; b < C::BPW; ++b)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    x0[b][i] = nt_load(&src[(unsigned)(j0 + 16 * g + c + (4 * b + i) * C::COLS)]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) transpose_lane_block(x0[0][i], x0[1][i], x0[2][i], x0[3][i]);
+        } else {
+#pragma unroll
+            for (int b = 0; b < C::BPW; ++b) {
+                const int blk = wave * C::BPW + b;
+                const int t = blk / (C::COLS / 16);
+                const int j = (blk % (C::COLS / 16)) * 16 + c;
+                const long long tg = t0 + t;
+                // loads past the last sequence are clamped to it; their results are not
+                // stored.  The input is read once, so it is loaded non-temporally.
+                const float2* src = a.in + (tg < a.ntrans ? tg : a.ntrans - 1) * C::L;
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    x0[b][i] = nt_load(&src[(unsigned)(j + (4 * g + i) * C::COLS)]);
+            }
         }
         // the twiddle table is loaded behind the sequences, so that a workgroup issues its
         // data loads first, and stored to the LDS at the end of this block; the later stages
@@ -312,6 +367,32 @@ __device__ __forceinline__ void mfma_stage(const MfmaArgs& a, float2* lds, float
     }
 
     if constexpr (!FIRST) __syncthreads();
+
+    if constexpr (LAST && C::ROWS) {
+        // the transpose between g and b gives lane (c, g) column 16 g + c of the wave's 64
+        // for rows 4 b + i: every store writes a contiguous 512-byte row piece
+        const int blk0 = wave * C::BPW;
+        const long long tg = t0 + blk0 / (C::COLS / 16);
+        const int j0 = (blk0 % (C::COLS / 16)) * 16;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) transpose_lane_block(y[0][i], y[1][i], y[2][i], y[3][i]);
+        if (tg < a.ntrans) {
+            float2* dst = a.out + tg * C::L;
+            // per-stage scaling: undo s0 and the 2^-4 of each later stage (one s0 per
+            // sequence); the inverse's conjugation of the output rides on the scale of Im
+            const float sc = C::SCALE == MFMA_SCALE_STAGE ? ldexpf(a.scale, 4 * (C::NS - 1) - sh[0]) : a.scale;
+            const float scy = a.conj_out ? -sc : sc;
+            with_nt(a.nt_out != 0, [&](auto ntc) {
+#pragma unroll
+                for (int b = 0; b < C::BPW; ++b)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        put<decltype(ntc)::value>(&dst[(unsigned)(j0 + 16 * g + c + (4 * b + i) * C::COLS)],
+                                                  float2{y[b][i].x * sc, y[b][i].y * scy});
+            });
+        }
+        return;
+    }
 
 #pragma unroll
     for (int b = 0; b < C::BPW; ++b) {
