@@ -51,10 +51,10 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 # rocprofv3 --pmc over `python bench.py --counters` (N = 2^24 FP32 pass kernels): average
 # FETCH_SIZE and WRITE_SIZE per dispatch in KiB, and the same counters for a 256 MiB device
 # copy (__amd_rocclr_copyBuffer) in that run.
-PMC_COPY_KIB = {"fetch": 131085.21, "write": 262144.0, "bytes_kib": 262144.0}
+PMC_COPY_KIB = {"fetch": 131085.31, "write": 262144.0, "bytes_kib": 262144.0}
 PMC_KIB = {
-    "tile_float_L256_nb16_wg256_truefalse": {"fetch": 65620.62, "write": 131072.0},
-    "tile_float_L256_nb16_wg256_truetrue": {"fetch": 65740.66, "write": 131072.0},
+    "tile_float_L256_nb16_wg256_truefalse": {"fetch": 65616.73, "write": 131072.0},
+    "tile_float_L256_nb16_wg256_truetrue": {"fetch": 65753.16, "write": 131072.0},
 }
 
 
